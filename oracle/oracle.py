@@ -1,0 +1,250 @@
+"""Python access to the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker or the timed CPU baseline.  The product path
+(asohf_amd + libasohf_hip) never imports it.
+
+  - the C restatement in asohf_oracle.c (libasohf_oracle.so), FP32 and FP64;
+  - run_ref(): the reference's own Fortran DENSITY / INTERPOLATE_DENSITY_TSC
+    compiled from the reference sources into _ref/asohf_ref_density
+    (built by `make ref` where the reference tree exists).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libasohf_oracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "asohf_ref_density")
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+_dp = C.POINTER(C.c_double)
+
+
+class OrcGeom(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("nl", C.c_int32),
+                ("npatch", _ip), ("pnx", _ip), ("pny", _ip), ("pnz", _ip),
+                ("prx", _fp), ("pry", _fp), ("prz", _fp),
+                ("dx", C.c_float), ("dy", C.c_float), ("dz", C.c_float),
+                ("lado0", C.c_float), ("rote", C.c_float), ("rete", C.c_float),
+                ("fdm", C.c_float), ("interp_degree", C.c_int32),
+                ("radx", _fp), ("rady", _fp), ("radz", _fp)]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "libasohf_oracle.so"], check=True)
+    if os.path.isdir("/root/reference"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        G = C.POINTER(OrcGeom)
+        L.orc_malla.restype = C.c_float
+        L.orc_malla.argtypes = [C.c_int, C.c_float, _fp]
+        L.orc_kernels32.argtypes = [C.c_int, C.c_int, _fp]
+        L.orc_kernels64.argtypes = [C.c_int, C.c_int, _dp]
+        for nm in ("orc_density_a32",):
+            getattr(L, nm).argtypes = [G, C.c_int64, _fp, _fp, _fp, _fp, _ip, _fp, _fp, C.c_int]
+        L.orc_density_a64.argtypes = [G, C.c_int64, _fp, _fp, _fp, _fp, _ip, _dp, _dp, _dp, C.c_int]
+        L.orc_density_b32.argtypes = [G, C.c_int64, _fp, _fp, _fp, _fp, _fp, _fp, C.c_int]
+        L.orc_density_b64.argtypes = [G, C.c_int64, _fp, _fp, _fp, _fp, _dp, _dp, _dp, C.c_int]
+        L.orc_adjoint_b64.argtypes = [G, C.c_int64, _fp, _fp, _fp, _fp, _dp, _dp,
+                                      _dp, _dp, _dp, _dp, C.c_int]
+        L.orc_pairs.restype = C.c_int64
+        L.orc_pairs.argtypes = [G, C.c_int, C.c_int64, _fp, _fp, _fp, _ip, _ip, C.c_int64]
+        L.orc_stencil_work_a.argtypes = [G, C.c_int64, _fp, _fp, _fp, _ip, C.POINTER(C.c_int64)]
+        for nm in ("orc_density_a32", "orc_density_a64", "orc_density_b32", "orc_density_b64",
+                   "orc_adjoint_b64"):
+            getattr(L, nm).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class _G:
+    """Keeps the numpy buffers behind an OrcGeom alive."""
+
+    def __init__(self, geom, use_radx: bool = True):
+        self.keep = [geom.npatch, geom.pnx, geom.pny, geom.pnz, geom.prx, geom.pry, geom.prz,
+                     geom.radx, geom.rady, geom.radz]
+        self.s = OrcGeom(geom.nx, geom.ny, geom.nz, geom.nl, _p(geom.npatch, _ip),
+                         _p(geom.pnx, _ip), _p(geom.pny, _ip), _p(geom.pnz, _ip),
+                         _p(geom.prx, _fp), _p(geom.pry, _fp), _p(geom.prz, _fp),
+                         geom.dx, geom.dy, geom.dz, geom.lado0, geom.rote, geom.rete,
+                         geom.fdm, geom.interp_degree,
+                         _p(geom.radx, _fp) if use_radx else None,
+                         _p(geom.rady, _fp) if use_radx else None,
+                         _p(geom.radz, _fp) if use_radx else None)
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _arrs(snap):
+    x, y, z, m = (np.ascontiguousarray(a, dtype=np.float32) for a in (snap.x, snap.y, snap.z, snap.m))
+    esp = np.ascontiguousarray(snap.npart_esp, dtype=np.int32)
+    return x, y, z, m, esp
+
+
+def malla(n: int, lado: float):
+    rad = np.zeros(n + 2, np.float32)
+    d = lib().orc_malla(n, np.float32(lado), _p(rad, _fp))
+    return rad, np.float32(d)
+
+
+def kernels32(nl: int, degree: int) -> np.ndarray:
+    kmax = 1 << nl
+    K = np.zeros((nl + 1, 2 * kmax + 1), np.float32)
+    lib().orc_kernels32(nl, degree, _p(K, _fp))
+    return K
+
+
+def kernels64(nl: int, degree: int) -> np.ndarray:
+    kmax = 1 << nl
+    K = np.zeros((nl + 1, 2 * kmax + 1), np.float64)
+    lib().orc_kernels64(nl, degree, _p(K, _dp))
+    return K
+
+
+def _out(geom, dt):
+    u1 = np.zeros((geom.nz, geom.ny, geom.nx), dt)
+    u11 = np.zeros(max(int(geom.patch_cells().sum()), 1), dt)
+    return u1, u11
+
+
+def density32(snap, mode: str = "B", nthreads: int = 1):
+    """Reference-faithful FP32 fields: (u1[nz,ny,nx], u11 packed)."""
+    g = _G(snap.geom)
+    x, y, z, m, esp = _arrs(snap)
+    u1, u11 = _out(snap.geom, np.float32)
+    L = lib()
+    if mode == "A":
+        rc = L.orc_density_a32(g.ref, len(x), _p(x, _fp), _p(y, _fp), _p(z, _fp), _p(m, _fp),
+                               _p(esp, _ip), _p(u1, _fp), _p(u11, _fp), nthreads)
+    else:
+        rc = L.orc_density_b32(g.ref, len(x), _p(x, _fp), _p(y, _fp), _p(z, _fp), _p(m, _fp),
+                               _p(u1, _fp), _p(u11, _fp), nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle rejected the input (rc={rc})")
+    return u1, u11[: int(snap.geom.patch_cells().sum())]
+
+
+def density64(snap, mode: str = "B", nthreads: int = 1):
+    """FP64 fields and deposited mass per grid: (u1, u11, grid_mass[1+P])."""
+    g = _G(snap.geom)
+    x, y, z, m, esp = _arrs(snap)
+    u1, u11 = _out(snap.geom, np.float64)
+    gm = np.zeros(1 + snap.geom.npatch_total, np.float64)
+    L = lib()
+    if mode == "A":
+        rc = L.orc_density_a64(g.ref, len(x), _p(x, _fp), _p(y, _fp), _p(z, _fp), _p(m, _fp),
+                               _p(esp, _ip), _p(u1, _dp), _p(u11, _dp), _p(gm, _dp), nthreads)
+    else:
+        rc = L.orc_density_b64(g.ref, len(x), _p(x, _fp), _p(y, _fp), _p(z, _fp), _p(m, _fp),
+                               _p(u1, _dp), _p(u11, _dp), _p(gm, _dp), nthreads)
+    if rc != 0:
+        raise ValueError(f"oracle rejected the input (rc={rc})")
+    return u1, u11[: int(snap.geom.patch_cells().sum())], gm
+
+
+def adjoint64(snap, g1: np.ndarray, g11: np.ndarray, nthreads: int = 0):
+    """FP64 gradients of L = sum(g1*u1) + sum(g11*u11) for mode B:
+    (dm, dx, dy, dz)."""
+    g = _G(snap.geom)
+    x, y, z, m, _ = _arrs(snap)
+    g1 = np.ascontiguousarray(g1, np.float64)
+    g11 = np.ascontiguousarray(g11 if g11.size else np.zeros(1), np.float64)
+    outs = [np.zeros(len(x), np.float64) for _ in range(4)]
+    rc = lib().orc_adjoint_b64(g.ref, len(x), _p(x, _fp), _p(y, _fp), _p(z, _fp), _p(m, _fp),
+                               _p(g1, _dp), _p(g11, _dp), *[_p(o, _dp) for o in outs], nthreads)
+    if rc != 0:
+        raise ValueError("oracle rejected the input")
+    return tuple(outs)
+
+
+def pairs(snap, mode: str = "B", all_members: bool = False) -> np.ndarray:
+    """Membership records [n_pairs, 5] = (grid, particle, ix, jy, kz)."""
+    g = _G(snap.geom)
+    x, y, z, _, esp = _arrs(snap)
+    md = 2 if mode == "B" else (3 if all_members else 1)
+    L = lib()
+    cnt = L.orc_pairs(g.ref, md, len(x), _p(x, _fp), _p(y, _fp), _p(z, _fp), _p(esp, _ip), None, 0)
+    out = np.zeros((max(cnt, 1), 5), np.int32)
+    L.orc_pairs(g.ref, md, len(x), _p(x, _fp), _p(y, _fp), _p(z, _fp), _p(esp, _ip),
+                _p(out, _ip), cnt)
+    return out[:cnt]
+
+
+def npairs(snap, mode: str = "B") -> int:
+    g = _G(snap.geom)
+    x, y, z, _, esp = _arrs(snap)
+    md = 2 if mode == "B" else 3
+    return int(lib().orc_pairs(g.ref, md, len(x), _p(x, _fp), _p(y, _fp), _p(z, _fp),
+                               _p(esp, _ip), None, 0))
+
+
+def stencil_work_a(snap) -> np.ndarray:
+    g = _G(snap.geom)
+    x, y, z, _, esp = _arrs(snap)
+    out = np.zeros(snap.geom.nl + 1, np.int64)
+    lib().orc_stencil_work_a(g.ref, len(x), _p(x, _fp), _p(y, _fp), _p(z, _fp), _p(esp, _ip),
+                             out.ctypes.data_as(C.POINTER(C.c_int64)))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the reference's own routines (oracle/_ref)
+# ---------------------------------------------------------------------------
+def ref_available() -> bool:
+    return os.path.exists(REF_BIN)
+
+
+def run_ref(snap, mode: str = "A"):
+    """Run the reference's DENSITY (mode A) or INTERPOLATE_DENSITY_TSC (mode B)
+    on this snapshot: returns (dx, dy, dz, radx, u1[nz,ny,nx], u11 packed)."""
+    geom = snap.geom
+    x, y, z, m, esp = _arrs(snap)
+    P = geom.npatch_total
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([1 if mode == "A" else 2, geom.nx, geom.ny, geom.nz, geom.nl, len(x),
+                      geom.interp_degree, P], np.int32).tofile(f)
+            geom.npatch.astype(np.int32).tofile(f)
+            esp.astype(np.int32).tofile(f)
+            np.array([geom.lado0, geom.rote, geom.rete, geom.fdm], np.float32).tofile(f)
+            for a in (geom.pnx, geom.pny, geom.pnz):
+                a.astype(np.int32).tofile(f)
+            for a in (geom.prx, geom.pry, geom.prz):
+                a.astype(np.float32).tofile(f)
+            for a in (x, y, z, m):
+                a.tofile(f)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([REF_BIN, fin, fout], check=True, stdout=subprocess.DEVNULL, env=env)
+        raw = np.fromfile(fout, np.float32)
+    d = raw[:3]
+    radx = raw[3:3 + geom.nx + 2]
+    o = 3 + geom.nx + 2
+    nc = geom.ncells_base
+    u1 = raw[o:o + nc].reshape(geom.nz, geom.ny, geom.nx)
+    u11 = raw[o + nc:]
+    return d[0], d[1], d[2], radx, u1, u11
