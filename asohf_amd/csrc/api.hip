@@ -1,0 +1,896 @@
+// api.hip -- host side of libasohf_hip: geometry planning, the batched and
+// plain pipelines, the adjoint, and the extern "C" entry points of
+// include/asohf_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/asohf_hip.h"
+#include "dev.h"
+#include "launch.h"
+
+using namespace asohf;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+struct DBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max<size_t>(bytes + bytes / 8, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) { p = nullptr; return e; }
+    cap = want;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// FP32 powers as the reference's REAL**INTEGER (binary powering)
+float powi_f(float a, int b) {
+  float r = 1.0f;
+  for (;;) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return r;
+}
+
+// MALLA (grids.f:2509-2573): RAD(0:N+1) for a side LADO
+void malla(int n, float lado, std::vector<float>& rad) {
+  rad.assign(n + 2, 0.f);
+  float a = -lado / 2.0f, b = lado / 2.0f;
+  float c = (b - a) / (float)(n - 1);
+  rad[1] = a;
+  for (int i = 2; i <= n; ++i) rad[i] = rad[1] + (float)(i - 1) * c;
+  rad[0] = rad[1] - c;
+  rad[n + 1] = rad[n] + c;
+}
+
+// COMPUTE_KERNELS (grids.f:997-1067) in FP32, the reference's operation
+// order, including the sequential normalisation sum.
+void compute_kernels(int nl, int degree, std::vector<float>& K) {
+  const int kmax = 1 << nl, kw = 2 * kmax + 1;
+  K.assign((size_t)kw * (nl + 1), 0.f);
+  for (int ir = 0; ir <= nl; ++ir) {
+    float* row = K.data() + (size_t)ir * kw + kmax;
+    const int n = 1 << ir;
+    const float h32 = (float)n + 0.5f;
+    const float h = 2.0f * h32 / 3.0f;
+    for (int i = -n; i <= n; ++i) {
+      if (degree == 2) {
+        float bas = (float)std::abs(i) / h;
+        if (bas <= 0.5f) row[i] = 0.75f - bas * bas;
+        else { float t = 1.5f - bas; row[i] = 0.5f * (t * t); }
+      } else {
+        float bas = (float)std::abs(i) / h32;
+        row[i] = 1.0f - bas;
+      }
+    }
+    float s = 0.0f;
+    for (int i = -n; i <= n; ++i)
+      for (int j = -n; j <= n; ++j)
+        for (int k = -n; k <= n; ++k) s = s + row[i] * row[j] * row[k];
+    s = 1.0f / powf(s, 1.0f / 3.0f);
+    for (int i = -n; i <= n; ++i) row[i] = s * row[i];
+  }
+}
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+  const unsigned char* c = (const unsigned char*)p;
+  for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+  return h;
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct HostPlan {
+  uint64_t key = 0;
+  bool valid = false;
+  std::vector<GridDesc> grids;
+  std::vector<int> tile_grid;
+  std::vector<uint64_t> hkeys;
+  std::vector<int2> hvals;
+  std::vector<int> hlist;
+  std::vector<float> radx, rady, radz, ktab;
+  Plan dev{};
+  long long ncells = 0;
+  long long u11_cells = 0;   // cells of the patch output array
+  long long u1_cells = 0;
+};
+
+struct asohf_ctx {
+  int device = 0;
+  int cus = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  asohf_tune tune{{16, 8, 8}, 256, 4096, 16};
+  HostPlan plan;
+  DBuf plan_blob;
+  DBuf cnt, start, cursor, bsum, counters, done, nitems, split, tmass, entries, items, scratch;
+  DBuf flags, idx;
+  DBuf h_in[4], h_out[2], h_mass;    // device copies for the host entry point
+  Counters* hcount = nullptr;          // pinned
+  unsigned long long* hscalar = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  asohf_stats stats{};
+  bool stats_pending = false;
+  std::map<std::pair<int, int>, std::vector<float>> ktab_cache;
+};
+
+static asohf_status fail(asohf_ctx* c, asohf_status st, const std::string& msg) {
+  if (c) c->err = msg;
+  return st;
+}
+static asohf_status hip_fail(asohf_ctx* c, hipError_t e, const char* what) {
+  return fail(c, e == hipErrorOutOfMemory ? ASOHF_ERR_OOM : ASOHF_ERR_HIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(expr, what)                         \
+  do {                                             \
+    hipError_t _e = (expr);                        \
+    if (_e != hipSuccess) return hip_fail(ctx, _e, what); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// geometry validation and planning
+// ---------------------------------------------------------------------------
+static asohf_status check_geom(asohf_ctx* ctx, const asohf_geom* g, int mode) {
+  if (!g) return fail(ctx, ASOHF_ERR_INVALID, "geometry is NULL");
+  if (g->nx < 3 || g->ny < 3 || g->nz < 3)
+    return fail(ctx, ASOHF_ERR_INVALID, "base grid needs at least 3 cells per axis");
+  if (g->nx > 16000 || g->ny > 16000 || g->nz > 16000)
+    return fail(ctx, ASOHF_ERR_CAPACITY, "base grid side above 16000 cells");
+  if (g->nl < 0 || g->nl > kNLevels) return fail(ctx, ASOHF_ERR_CAPACITY, "NL outside 0..9 (NLEVELS)");
+  if (!g->npatch) return fail(ctx, ASOHF_ERR_INVALID, "npatch is NULL");
+  long long P = 0;
+  for (int l = 0; l <= g->nl; ++l) {
+    if (g->npatch[l] < 0) return fail(ctx, ASOHF_ERR_INVALID, "negative NPATCH entry");
+    P += g->npatch[l];
+  }
+  if (P > (1 << 26)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^26 patches");
+  if (P > 0 && (!g->patchnx || !g->patchny || !g->patchnz || !g->patchrx || !g->patchry || !g->patchrz))
+    return fail(ctx, ASOHF_ERR_INVALID, "patch arrays are NULL");
+  for (long long p = 0; p < P; ++p) {
+    const int a = g->patchnx[p], b = g->patchny[p], c = g->patchnz[p];
+    if (a < 1 || b < 1 || c < 1) return fail(ctx, ASOHF_ERR_INVALID, "patch with a non-positive side");
+    if (a > 16000 || b > 16000 || c > 16000) return fail(ctx, ASOHF_ERR_CAPACITY, "patch side above 16000 cells");
+    if (!std::isfinite(g->patchrx[p]) || !std::isfinite(g->patchry[p]) || !std::isfinite(g->patchrz[p]))
+      return fail(ctx, ASOHF_ERR_INVALID, "non-finite patch origin");
+  }
+  if (!(g->dx > 0 && g->dy > 0 && g->dz > 0 && g->lado0 > 0))
+    return fail(ctx, ASOHF_ERR_INVALID, "cell sizes and LADO0 must be positive");
+  if (!(g->rote > 0 && g->rete > 0 && g->fdm > 0) || !std::isfinite(g->rote) || !std::isfinite(g->rete))
+    return fail(ctx, ASOHF_ERR_INVALID, "ROTE, RETE and FDM must be positive");
+  if (mode != ASOHF_MODE_TABLE && mode != ASOHF_MODE_TSC) return fail(ctx, ASOHF_ERR_INVALID, "unknown mode");
+  if (mode == ASOHF_MODE_TABLE && g->interp_degree != 1 && g->interp_degree != 2)
+    return fail(ctx, ASOHF_ERR_INVALID, "interp_degree must be 1 or 2");
+  return ASOHF_OK;
+}
+
+static uint64_t plan_key(const asohf_geom* g, int mode, const asohf_tune& tn, const asohf_opts* o) {
+  uint64_t h = 1469598103934665603ull;
+  long long P = 0;
+  for (int l = 0; l <= g->nl; ++l) P += g->npatch[l];
+  int hdr[16] = {g->nx, g->ny, g->nz, g->nl, mode, g->interp_degree, tn.tile[0], tn.tile[1], tn.tile[2],
+                 o->u1_ld[0], o->u1_ld[1], o->u11_ld[0], o->u11_ld[1], o->u11_ld[2], 0, 0};
+  h = fnv(h, hdr, sizeof(hdr));
+  float sc[7] = {g->dx, g->dy, g->dz, g->lado0, g->rote, g->rete, g->fdm};
+  h = fnv(h, sc, sizeof(sc));
+  h = fnv(h, g->npatch, sizeof(int32_t) * (g->nl + 1));
+  if (P) {
+    h = fnv(h, g->patchnx, 4 * P); h = fnv(h, g->patchny, 4 * P); h = fnv(h, g->patchnz, 4 * P);
+    h = fnv(h, g->patchrx, 4 * P); h = fnv(h, g->patchry, 4 * P); h = fnv(h, g->patchrz, 4 * P);
+  }
+  const float* r[3] = {g->radx, g->rady, g->radz};
+  const int n[3] = {g->nx, g->ny, g->nz};
+  for (int a = 0; a < 3; ++a) {
+    if (r[a]) h = fnv(h, r[a], sizeof(float) * (n[a] + 2));
+    else h = fnv(h, "null", 4);
+  }
+  return h;
+}
+
+static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, const asohf_opts* o) {
+  const asohf_tune& tn = ctx->tune;
+  const uint64_t key = plan_key(g, mode, tn, o);
+  HostPlan& H = ctx->plan;
+  if (H.valid && H.key == key) return ASOHF_OK;
+  H = HostPlan();
+  const int nl = g->nl;
+  long long P = 0;
+  std::vector<int> level_start(nl + 2, 0);
+  for (int l = 0; l <= nl; ++l) { level_start[l] = (int)P; P += g->npatch[l]; }
+  level_start[nl + 1] = (int)P;
+  const int T[3] = {tn.tile[0], tn.tile[1], tn.tile[2]};
+
+  // cell centres: COMMON /GRID/ or MALLA with LADO = LADO0 - LADO0/NX (asohf.f:323)
+  const float lado = g->lado0 - g->lado0 / (float)g->nx;
+  if (g->radx) H.radx.assign(g->radx, g->radx + g->nx + 2); else malla(g->nx, lado, H.radx);
+  if (g->rady) H.rady.assign(g->rady, g->rady + g->ny + 2); else malla(g->ny, lado, H.rady);
+  if (g->radz) H.radz.assign(g->radz, g->radz + g->nz + 2); else malla(g->nz, lado, H.radz);
+  // kernel table (table mode), cached per (nl, degree)
+  {
+    auto k = std::make_pair(nl, g->interp_degree);
+    auto it = ctx->ktab_cache.find(k);
+    if (it == ctx->ktab_cache.end()) {
+      std::vector<float> K;
+      compute_kernels(nl, g->interp_degree == 1 ? 1 : 2, K);
+      it = ctx->ktab_cache.emplace(k, std::move(K)).first;
+    }
+    H.ktab = it->second;
+  }
+  const float r3 = g->rete * (g->rete * g->rete);   // RETE**3
+  // ---- grids ----
+  H.grids.resize(1 + P);
+  int ntiles = 0;
+  long long u11_off = 0;
+  const bool padded = o->u11_ld[0] > 0;
+  {
+    GridDesc& B = H.grids[0];
+    memset(&B, 0, sizeof(B));
+    B.n[0] = g->nx; B.n[1] = g->ny; B.n[2] = g->nz;
+    B.level = 0;
+    for (int a = 0; a < 3; ++a) {
+      B.nt[a] = (B.n[a] + T[a] - 1) / T[a];
+      B.org[a] = -g->lado0 / 2.0f;
+    }
+    B.d[0] = g->dx; B.d[1] = g->dy; B.d[2] = g->dz;
+    B.den = g->dx * g->dy * g->dz * g->rote * r3;         // DENBAS (grids.f:824)
+    B.tile0 = 0;
+    ntiles = B.nt[0] * B.nt[1] * B.nt[2];
+    const int l0 = o->u1_ld[0] > 0 ? o->u1_ld[0] : g->nx;
+    const int l1 = o->u1_ld[1] > 0 ? o->u1_ld[1] : g->ny;
+    B.ld1 = l0;
+    B.ld2 = l0 * l1;
+    B.off = 0;
+    H.u1_cells = (long long)l0 * l1 * g->nz;
+  }
+  for (int l = 0; l <= nl; ++l) {
+    const float p2 = powi_f(2.0f, l);
+    const float dxpa = g->dx / p2, dypa = g->dy / p2, dzpa = g->dz / p2;
+    const float den = dxpa * dypa * dzpa * g->rote * r3;   // DENBAS of the level (grids.f:844)
+    for (int p = level_start[l]; p < level_start[l + 1]; ++p) {
+      GridDesc& G = H.grids[1 + p];
+      memset(&G, 0, sizeof(G));
+      G.n[0] = g->patchnx[p]; G.n[1] = g->patchny[p]; G.n[2] = g->patchnz[p];
+      G.level = l;
+      G.d[0] = dxpa; G.d[1] = dypa; G.d[2] = dzpa;
+      G.den = den;
+      const float pr[3] = {g->patchrx[p], g->patchry[p], g->patchrz[p]};
+      const float d[3] = {dxpa, dypa, dzpa};
+      const float dbase = g->dx;   // MAXKERNELEXT = DX on every axis (grids.f:837, 880-885)
+      for (int a = 0; a < 3; ++a) {
+        G.pr[a] = pr[a];
+        if (mode == ASOHF_MODE_TABLE) {
+          const float xlfix = pr[a] - d[a];                        // grids.f:871-873
+          const float xr = xlfix + (float)G.n[a] * d[a];           // grids.f:878-880
+          G.org[a] = xlfix;
+          G.lo[a] = xlfix - dbase;                                 // grids.f:881-886
+          G.hi[a] = xr + dbase;
+        } else {
+          const float xl = pr[a] - 2.0f * d[a];                    // (1+BUF)*DXPA, grids.f:1294
+          G.org[a] = xl;
+          G.lo[a] = xl;
+          G.hi[a] = xl + (float)(G.n[a] + 2) * d[a];               // (N1+2*BUF)*DXPA, grids.f:1300
+        }
+      }
+      if (padded) {
+        G.ld1 = o->u11_ld[0];
+        G.ld2 = o->u11_ld[0] * o->u11_ld[1];
+        G.off = (long long)p * o->u11_ld[0] * o->u11_ld[1] * o->u11_ld[2];
+      } else {
+        G.ld1 = G.n[0];
+        G.ld2 = G.n[0] * G.n[1];
+        G.off = u11_off;
+      }
+      u11_off += (long long)G.n[0] * G.n[1] * G.n[2];
+      H.ncells += (long long)G.n[0] * G.n[1] * G.n[2];
+      if (l == 0) {   // patches numbered before level 1 are never deposited (grids.f:846)
+        G.tile0 = ntiles;
+        continue;
+      }
+      for (int a = 0; a < 3; ++a) G.nt[a] = (G.n[a] + T[a] - 1) / T[a];
+      G.tile0 = ntiles;
+      const long long nt = (long long)G.nt[0] * G.nt[1] * G.nt[2];
+      if ((long long)ntiles + nt > (1ll << 30)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^30 tiles");
+      ntiles += (int)nt;
+    }
+  }
+  H.u11_cells = padded ? P * (long long)o->u11_ld[0] * o->u11_ld[1] * o->u11_ld[2] : u11_off;
+  if (padded) {
+    for (long long p = 0; p < P; ++p)
+      if (g->patchnx[p] > o->u11_ld[0] || g->patchny[p] > o->u11_ld[1] || g->patchnz[p] > o->u11_ld[2])
+        return fail(ctx, ASOHF_ERR_CAPACITY, "patch larger than NAMRX/NAMRY/NAMRZ of the padded layout");
+  }
+  if (o->u1_ld[0] > 0 && (o->u1_ld[0] < g->nx || o->u1_ld[1] < g->ny))
+    return fail(ctx, ASOHF_ERR_CAPACITY, "NX/NY above the leading dimensions of U1");
+  H.ncells += (long long)g->nx * g->ny * g->nz;
+  H.tile_grid.resize(ntiles);
+  for (size_t gi = 0; gi < H.grids.size(); ++gi) {
+    const GridDesc& G = H.grids[gi];
+    if (gi > 0 && G.level == 0) continue;
+    const int nt = G.nt[0] * G.nt[1] * G.nt[2];
+    for (int k = 0; k < nt; ++k) H.tile_grid[G.tile0 + k] = (int)gi;
+  }
+  // ---- per-level spatial hash of the membership boxes ----
+  Plan& D = H.dev;
+  memset(&D, 0, sizeof(D));
+  const float x0 = -g->lado0 / 2.0f;
+  std::vector<std::pair<uint64_t, int>> ent;
+  for (int l = 1; l <= nl; ++l) {
+    LevelHash& lh = D.lh[l];
+    const int lo = level_start[l], hi = level_start[l + 1];
+    if (hi == lo) { lh.x0 = x0; lh.inv_bs = 1.0f / g->lado0; lh.nb = 1; continue; }
+    std::vector<float> w;
+    w.reserve(hi - lo);
+    for (int p = lo; p < hi; ++p) {
+      const GridDesc& G = H.grids[1 + p];
+      w.push_back(std::max({G.hi[0] - G.lo[0], G.hi[1] - G.lo[1], G.hi[2] - G.lo[2]}));
+    }
+    std::nth_element(w.begin(), w.begin() + w.size() / 2, w.end());
+    const double bs = std::max<double>(w[w.size() / 2], g->lado0 / 16384.0);
+    int nb = (int)std::ceil(g->lado0 / bs);
+    nb = std::min(std::max(nb, 1), 16384);
+    lh.x0 = x0;
+    lh.inv_bs = (float)((double)nb / (double)g->lado0);
+    lh.nb = nb;
+    for (int p = lo; p < hi; ++p) {
+      const GridDesc& G = H.grids[1 + p];
+      int b0[3], b1[3];
+      for (int a = 0; a < 3; ++a) {
+        double t0 = ((double)G.lo[a] - (double)lh.x0) * (double)lh.inv_bs - 0.01;
+        double t1 = ((double)G.hi[a] - (double)lh.x0) * (double)lh.inv_bs + 0.01;
+        t0 = std::min(std::max(t0, -1.0), 1.0e6);
+        t1 = std::min(std::max(t1, -1.0), 1.0e6);
+        b0[a] = std::min(std::max((int)std::floor(t0), 0), nb - 1);
+        b1[a] = std::min(std::max((int)std::floor(t1), 0), nb - 1);
+      }
+      for (int c = b0[2]; c <= b1[2]; ++c)
+        for (int b = b0[1]; b <= b1[1]; ++b)
+          for (int a = b0[0]; a <= b1[0]; ++a) ent.emplace_back(bucket_key(l, a, b, c), 1 + p);
+    }
+  }
+  std::sort(ent.begin(), ent.end());
+  std::vector<uint64_t> ukeys;
+  std::vector<int2> uvals;
+  H.hlist.resize(ent.size());
+  for (size_t i = 0; i < ent.size(); ++i) {
+    H.hlist[i] = ent[i].second;
+    if (i == 0 || ent[i].first != ent[i - 1].first) {
+      ukeys.push_back(ent[i].first);
+      uvals.push_back(make_int2((int)i, 0));
+    }
+    uvals.back().y++;
+  }
+  size_t cap = 16;
+  while (cap < 2 * ukeys.size() + 2) cap <<= 1;
+  H.hkeys.assign(cap, kHashEmpty);
+  H.hvals.assign(cap, make_int2(0, 0));
+  for (size_t i = 0; i < ukeys.size(); ++i) {
+    unsigned s = (unsigned)hash_mix(ukeys[i]) & (unsigned)(cap - 1);
+    while (H.hkeys[s] != kHashEmpty) s = (s + 1) & (unsigned)(cap - 1);
+    H.hkeys[s] = ukeys[i];
+    H.hvals[s] = uvals[i];
+  }
+  if (H.hlist.empty()) H.hlist.push_back(0);
+  // ---- upload ----
+  size_t off = 0;
+  auto place = [&](size_t bytes) { size_t o2 = off; off = align_up(off + bytes, 256); return o2; };
+  const size_t o_grids = place(sizeof(GridDesc) * H.grids.size());
+  const size_t o_tg = place(sizeof(int) * std::max<size_t>(H.tile_grid.size(), 1));
+  const size_t o_hk = place(sizeof(uint64_t) * H.hkeys.size());
+  const size_t o_hv = place(sizeof(int2) * H.hvals.size());
+  const size_t o_hl = place(sizeof(int) * H.hlist.size());
+  const size_t o_rx = place(sizeof(float) * H.radx.size());
+  const size_t o_ry = place(sizeof(float) * H.rady.size());
+  const size_t o_rz = place(sizeof(float) * H.radz.size());
+  const size_t o_kt = place(sizeof(float) * H.ktab.size());
+  HIPCHK(ctx->plan_blob.ensure(off), "allocating the geometry plan");
+  char* base = ctx->plan_blob.as<char>();
+  auto up = [&](size_t o2, const void* src, size_t bytes) {
+    return hipMemcpyAsync(base + o2, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+  };
+  HIPCHK(up(o_grids, H.grids.data(), sizeof(GridDesc) * H.grids.size()), "uploading grids");
+  if (!H.tile_grid.empty()) HIPCHK(up(o_tg, H.tile_grid.data(), sizeof(int) * H.tile_grid.size()), "uploading tiles");
+  HIPCHK(up(o_hk, H.hkeys.data(), sizeof(uint64_t) * H.hkeys.size()), "uploading hash");
+  HIPCHK(up(o_hv, H.hvals.data(), sizeof(int2) * H.hvals.size()), "uploading hash");
+  HIPCHK(up(o_hl, H.hlist.data(), sizeof(int) * H.hlist.size()), "uploading hash");
+  HIPCHK(up(o_rx, H.radx.data(), sizeof(float) * H.radx.size()), "uploading RADX");
+  HIPCHK(up(o_ry, H.rady.data(), sizeof(float) * H.rady.size()), "uploading RADY");
+  HIPCHK(up(o_rz, H.radz.data(), sizeof(float) * H.radz.size()), "uploading RADZ");
+  HIPCHK(up(o_kt, H.ktab.data(), sizeof(float) * H.ktab.size()), "uploading kernel table");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "uploading the plan");
+  D.grids = (const GridDesc*)(base + o_grids);
+  D.tile_grid = (const int*)(base + o_tg);
+  D.hkeys = (const uint64_t*)(base + o_hk);
+  D.hvals = (const int2*)(base + o_hv);
+  D.hlist = (const int*)(base + o_hl);
+  D.radx = (const float*)(base + o_rx);
+  D.rady = (const float*)(base + o_ry);
+  D.radz = (const float*)(base + o_rz);
+  D.ktab = (const float*)(base + o_kt);
+  D.nl = nl;
+  D.ngrids = (int)H.grids.size();
+  D.ntiles = ntiles;
+  D.kmax = 1 << nl;
+  D.kw = 2 * D.kmax + 1;
+  D.hmask = (unsigned)(cap - 1);
+  for (int a = 0; a < 3; ++a) D.tile[a] = T[a];
+  D.mode = mode;
+  D.xl = -g->lado0 / 2.0f;
+  D.fdm = g->fdm;
+  H.key = key;
+  H.valid = true;
+  return ASOHF_OK;
+}
+
+static asohf_opts default_opts() {
+  asohf_opts o;
+  memset(&o, 0, sizeof(o));
+  o.mode = ASOHF_MODE_TSC;
+  o.reduction = ASOHF_REDUCE_DETERMINISTIC;
+  o.path = ASOHF_PATH_BATCHED;
+  return o;
+}
+
+static asohf_status make_parts(asohf_ctx* ctx, const asohf_opts& o, int64_t n, const float* rx,
+                               const float* ry, const float* rz, const float* m, const int32_t* esp,
+                               Parts& A) {
+  if (n < 0) return fail(ctx, ASOHF_ERR_INVALID, "negative particle count");
+  if (n > 0 && (!rx || !ry || !rz || !m)) return fail(ctx, ASOHF_ERR_INVALID, "particle arrays are NULL");
+  memset(&A, 0, sizeof(A));
+  A.x = rx; A.y = ry; A.z = rz; A.m = m;
+  A.n = n;
+  if (o.mode == ASOHF_MODE_TABLE) {
+    if (!esp) return fail(ctx, ASOHF_ERR_INVALID, "npart_esp is NULL (table mode)");
+    long long s = 0;
+    A.sp[0] = 0;
+    for (int k = 0; k < kNEsp; ++k) {
+      if (esp[k] < 0) return fail(ctx, ASOHF_ERR_INVALID, "negative NPART_ESP entry");
+      s += esp[k];
+      A.sp[k + 1] = s;
+    }
+    if (s > n) return fail(ctx, ASOHF_ERR_INVALID, "SUM(NPART_ESP) exceeds the particle count");
+    A.n = s;   // DENSITY deposits particles 1..SUM(NPART_ESP) (grids.f:784-785)
+  }
+  return ASOHF_OK;
+}
+
+static asohf_status decode_err(asohf_ctx* ctx, unsigned e) {
+  if (!e) return ASOHF_OK;
+  std::string m = "invalid particle data:";
+  if (e & kErrNonFinite) m += " non-finite position or mass;";
+  if (e & kErrNegMass) m += " negative mass;";
+  if (e & kErrWrap) m += " base-grid cell index outside the single periodic wrap range of DENSITY;";
+  return fail(ctx, ASOHF_ERR_INVALID, m);
+}
+
+// ---------------------------------------------------------------------------
+// batched pipeline
+// ---------------------------------------------------------------------------
+static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, float* u1, float* u11,
+                                double* grid_mass, hipStream_t s) {
+  HostPlan& H = ctx->plan;
+  const Plan& P = H.dev;
+  const int ntiles = P.ntiles;
+  const long long nkeys = 2ll * ntiles;
+  const int T = P.tile[0] * P.tile[1] * P.tile[2];
+  const int red = o.reduction;
+  HIPCHK(ctx->cnt.ensure(sizeof(unsigned) * std::max<long long>(nkeys, 1)), "allocating counts");
+  HIPCHK(ctx->start.ensure(sizeof(unsigned long long) * (nkeys + 1)), "allocating offsets");
+  HIPCHK(ctx->cursor.ensure(sizeof(unsigned long long) * std::max<long long>(nkeys, 1)), "allocating cursors");
+  HIPCHK(ctx->bsum.ensure(sizeof(unsigned long long) * scan_scratch_elems(nkeys)), "allocating scan");
+  HIPCHK(ctx->counters.ensure(sizeof(Counters)), "allocating counters");
+  HIPCHK(ctx->done.ensure(sizeof(unsigned) * std::max(ntiles, 1)), "allocating tile state");
+  HIPCHK(ctx->nitems.ensure(sizeof(int) * std::max(ntiles, 1)), "allocating tile state");
+  HIPCHK(ctx->split.ensure(sizeof(int) * std::max(ntiles, 1)), "allocating tile state");
+  HIPCHK(ctx->tmass.ensure(sizeof(double) * std::max(ntiles, 1)), "allocating tile sums");
+  Counters* C = ctx->counters.as<Counters>();
+
+  HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
+  HIPCHK(launch_fill_u32((unsigned*)C, 0u, sizeof(Counters) / 4, s), "clearing counters");
+  HIPCHK(launch_fill_u32(ctx->cnt.as<unsigned>(), 0u, nkeys, s), "clearing counts");
+  HIPCHK(launch_fill_u32(ctx->done.as<unsigned>(), 0u, ntiles, s), "clearing tile state");
+  HIPCHK(launch_count(P, A, ctx->cnt.as<unsigned>(), C, ctx->cus, s), "count kernel");
+  HIPCHK(launch_scan(ctx->cnt.as<unsigned>(), nkeys, ctx->bsum.as<unsigned long long>(),
+                     ctx->start.as<unsigned long long>(), ctx->cursor.as<unsigned long long>(), s),
+         "scan");
+  const int chunk = std::max(ctx->tune.chunk, 64);
+  const int chunk_wide = std::max(chunk / 64, 4);
+  HIPCHK(launch_plan_items(ctx->start.as<unsigned long long>(), ntiles, chunk, chunk_wide, 0, nullptr,
+                           ctx->nitems.as<int>(), ctx->split.as<int>(), C, s),
+         "planning items");
+  HIPCHK(hipMemcpyAsync(ctx->hcount, C, sizeof(Counters), hipMemcpyDeviceToHost, s), "reading counters");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar, ctx->start.as<unsigned long long>() + nkeys, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s), "reading totals");
+  HIPCHK(hipStreamSynchronize(s), "count pass");
+  const Counters hc = *ctx->hcount;
+  const unsigned long long n_entries = *ctx->hscalar;
+  asohf_status st = decode_err(ctx, hc.err);
+  if (st != ASOHF_OK) return st;
+  // fixed-point scale: a power of two with max(m) * scale <= 2^30, so every
+  // quantised term fits an int32 and any sum of <= 2^33 particles an int64
+  const float mmax = __builtin_bit_cast(float, hc.mmax_bits);
+  int e2 = 0;
+  if (mmax > 0.f) (void)std::frexp(mmax, &e2);
+  const double scale = std::ldexp(1.0, 30 - e2);
+  if (A.n > (1ll << 33)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^33 particles");
+  HIPCHK(ctx->entries.ensure(sizeof(int4) * std::max<unsigned long long>(n_entries, 1)), "allocating entries");
+  HIPCHK(ctx->items.ensure(sizeof(Item) * std::max<unsigned>(hc.n_items, 1)), "allocating work items");
+  HIPCHK(ctx->scratch.ensure((size_t)std::max<unsigned>(hc.n_split, 1) * T * (red == 0 ? 8 : 4)),
+         "allocating split-tile scratch");
+  HIPCHK(launch_scatter(P, A, ctx->cursor.as<unsigned long long>(), ctx->entries.as<int4>(), n_entries,
+                        ctx->cus, s),
+         "scatter kernel");
+  HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
+  HIPCHK(launch_plan_items(ctx->start.as<unsigned long long>(), ntiles, chunk, chunk_wide, 1,
+                           ctx->items.as<Item>(), ctx->nitems.as<int>(), ctx->split.as<int>(), C, s),
+         "writing items");
+  HIPCHK(launch_zero_split(ctx->scratch.p, red, T, (long long)std::max<unsigned>(hc.n_split, 1) * T, C, s),
+         "clearing scratch");
+  DepArgs D;
+  memset(&D, 0, sizeof(D));
+  D.entries = ctx->entries.as<int4>();
+  D.items = ctx->items.as<Item>();
+  D.nitems_tile = ctx->nitems.as<int>();
+  D.split_slot = ctx->split.as<int>();
+  D.done = ctx->done.as<unsigned>();
+  D.scratch = ctx->scratch.p;
+  D.C = C;
+  D.u1 = u1;
+  D.u11 = u11;
+  D.tile_mass_i = ctx->tmass.as<long long>();
+  D.tile_mass_d = ctx->tmass.as<double>();
+  D.scale = scale;
+  D.inv_scale = 1.0 / scale;
+  const int waves = std::max(ctx->tune.waves_per_cu, 1);
+  const int block = ctx->tune.block;
+  int grid = std::max(1, ctx->cus * waves / std::max(block / 64, 1));
+  grid = (int)std::min<long long>(grid, std::max<unsigned>(hc.n_items, 1));
+  HIPCHK(launch_deposit(P, D, red, grid, block, s), "deposit kernel");
+  if (grid_mass) HIPCHK(launch_grid_sums(P, D, red, grid_mass, s), "grid sums");
+  HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
+  ctx->stats = asohf_stats{};
+  ctx->stats.n_particles = A.n;
+  ctx->stats.n_pairs = (int64_t)hc.n_pairs;
+  ctx->stats.n_entries = (int64_t)n_entries;
+  ctx->stats.n_tiles = ntiles;
+  ctx->stats.n_items = hc.n_items;
+  ctx->stats.n_cells = H.ncells;
+  ctx->stats_pending = true;
+  return ASOHF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plain form: one launch per grid
+// ---------------------------------------------------------------------------
+static asohf_status run_plain(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, const asohf_geom* g,
+                              float* u1, float* u11, double* grid_mass, hipStream_t s) {
+  (void)o;
+  const Plan& P = ctx->plan.dev;
+  HIPCHK(ctx->counters.ensure(sizeof(Counters)), "allocating counters");
+  Counters* C = ctx->counters.as<Counters>();
+  HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
+  HIPCHK(launch_fill_u32((unsigned*)C, 0u, sizeof(Counters) / 4, s), "clearing counters");
+  HIPCHK(launch_plain_validate(P, A, C, ctx->cus, s), "validation");
+  HIPCHK(hipMemcpyAsync(ctx->hcount, C, sizeof(Counters), hipMemcpyDeviceToHost, s), "reading counters");
+  HIPCHK(hipStreamSynchronize(s), "validation");
+  asohf_status st = decode_err(ctx, ctx->hcount->err);
+  if (st != ASOHF_OK) return st;
+  HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
+  HIPCHK(launch_plain_zero(P, u1, u11, s), "zeroing");
+  HIPCHK(launch_plain_base(P, A, u1, ctx->cus, s), "base grid");
+  if (g->nl > 0) {
+    const long long n = A.n;
+    HIPCHK(ctx->flags.ensure(sizeof(unsigned) * std::max<long long>(n, 1)), "allocating flags");
+    HIPCHK(ctx->start.ensure(sizeof(unsigned long long) * (n + 1)), "allocating offsets");
+    HIPCHK(ctx->bsum.ensure(sizeof(unsigned long long) * scan_scratch_elems(n)), "allocating scan");
+    HIPCHK(ctx->idx.ensure(sizeof(long long) * std::max<long long>(n, 1)), "allocating candidates");
+    int p0 = 0;
+    for (int l = 0; l < 1; ++l) p0 += g->npatch[l];
+    for (int l = 1; l <= g->nl; ++l) {
+      if (g->npatch[l] == 0) continue;
+      HIPCHK(launch_plain_flags(P, A, l, ctx->flags.as<unsigned>(), ctx->cus, s), "level flags");
+      HIPCHK(launch_scan(ctx->flags.as<unsigned>(), n, ctx->bsum.as<unsigned long long>(),
+                         ctx->start.as<unsigned long long>(), nullptr, s),
+             "level scan");
+      HIPCHK(launch_plain_compact(ctx->flags.as<unsigned>(), ctx->start.as<unsigned long long>(), n,
+                                  ctx->idx.as<long long>(), ctx->cus, s),
+             "level compaction");
+      for (int p = p0; p < p0 + g->npatch[l]; ++p)
+        HIPCHK(launch_plain_patch(P, A, 1 + p, ctx->idx.as<long long>(), ctx->start.as<unsigned long long>() + n,
+                                  n, u11, ctx->cus, s),
+               "patch kernel");
+      p0 += g->npatch[l];
+    }
+  }
+  HIPCHK(launch_plain_finish(P, u1, u11, grid_mass, s), "normalisation");
+  HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
+  ctx->stats = asohf_stats{};
+  ctx->stats.n_particles = A.n;
+  ctx->stats.n_tiles = 0;
+  ctx->stats.n_cells = ctx->plan.ncells;
+  ctx->stats_pending = true;
+  return ASOHF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* asohf_hip_version(void) { return "asohf_hip 0.1 (gfx950)"; }
+
+asohf_status asohf_hip_create(asohf_ctx** out, int32_t device) {
+  if (!out) return ASOHF_ERR_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return ASOHF_ERR_NODEVICE;
+  asohf_ctx* ctx = new asohf_ctx();
+  if (device < 0) {
+    if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return ASOHF_ERR_HIP; }
+  } else {
+    if (device >= ndev) { delete ctx; return ASOHF_ERR_NODEVICE; }
+    ctx->device = device;
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) { delete ctx; return ASOHF_ERR_HIP; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) { delete ctx; return ASOHF_ERR_HIP; }
+  ctx->cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return ASOHF_ERR_HIP; }
+  for (auto& e : ctx->ev)
+    if (hipEventCreate(&e) != hipSuccess) { asohf_hip_destroy(ctx); return ASOHF_ERR_HIP; }
+  if (hipHostMalloc((void**)&ctx->hcount, sizeof(Counters)) != hipSuccess ||
+      hipHostMalloc((void**)&ctx->hscalar, sizeof(unsigned long long) * 4) != hipSuccess) {
+    asohf_hip_destroy(ctx);
+    return ASOHF_ERR_OOM;
+  }
+  *out = ctx;
+  return ASOHF_OK;
+}
+
+void asohf_hip_destroy(asohf_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DBuf* bufs[] = {&ctx->plan_blob, &ctx->cnt, &ctx->start, &ctx->cursor, &ctx->bsum, &ctx->counters,
+                  &ctx->done, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries, &ctx->items,
+                  &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1], &ctx->h_in[2],
+                  &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass};
+  for (DBuf* b : bufs) b->release();
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->hcount) (void)hipHostFree(ctx->hcount);
+  if (ctx->hscalar) (void)hipHostFree(ctx->hscalar);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* asohf_hip_last_error(const asohf_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+asohf_status asohf_hip_density(asohf_ctx* ctx, const asohf_geom* geom, int64_t n, const float* rx,
+                               const float* ry, const float* rz, const float* m, const int32_t* npart_esp,
+                               float* u1, float* u11, double* grid_mass, const asohf_opts* opts) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  const asohf_opts o = opts ? *opts : default_opts();
+  if (o.reduction != 0 && o.reduction != 1) return fail(ctx, ASOHF_ERR_INVALID, "unknown reduction");
+  if (o.path != 0 && o.path != 1) return fail(ctx, ASOHF_ERR_INVALID, "unknown path");
+  asohf_status st = check_geom(ctx, geom, o.mode);
+  if (st != ASOHF_OK) return st;
+  if (!u1) return fail(ctx, ASOHF_ERR_INVALID, "u1 is NULL");
+  long long P = 0;
+  for (int l = 0; l <= geom->nl; ++l) P += geom->npatch[l];
+  if (P > 0 && !u11) return fail(ctx, ASOHF_ERR_INVALID, "u11 is NULL but there are patches");
+  if (!tile_shape_supported(ctx->tune.tile)) return fail(ctx, ASOHF_ERR_INVALID, "unsupported tile shape");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  Parts A;
+  st = make_parts(ctx, o, n, rx, ry, rz, m, npart_esp, A);
+  if (st != ASOHF_OK) return st;
+  st = build_plan(ctx, geom, o.mode, &o);
+  if (st != ASOHF_OK) return st;
+  hipStream_t s = o.stream ? (hipStream_t)o.stream : ctx->stream;
+  if (o.path == ASOHF_PATH_PLAIN) return run_plain(ctx, o, A, geom, u1, u11, grid_mass, s);
+  return run_batched(ctx, o, A, u1, u11, grid_mass, s);
+}
+
+asohf_status asohf_hip_density_host(asohf_ctx* ctx, const asohf_geom* geom, int64_t n, const float* rx,
+                                    const float* ry, const float* rz, const float* m,
+                                    const int32_t* npart_esp, float* u1, float* u11, double* grid_mass,
+                                    const asohf_opts* opts) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  asohf_opts o = opts ? *opts : default_opts();
+  asohf_status st = check_geom(ctx, geom, o.mode);
+  if (st != ASOHF_OK) return st;
+  if (n < 0 || (n > 0 && (!rx || !ry || !rz || !m)) || !u1)
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL particle or output array");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  st = build_plan(ctx, geom, o.mode, &o);
+  if (st != ASOHF_OK) return st;
+  const HostPlan& H = ctx->plan;
+  hipStream_t s = o.stream ? (hipStream_t)o.stream : ctx->stream;
+  const size_t pb = sizeof(float) * std::max<int64_t>(n, 1);
+  const float* src[4] = {rx, ry, rz, m};
+  for (int k = 0; k < 4; ++k) {
+    HIPCHK(ctx->h_in[k].ensure(pb), "allocating particles");
+    if (n > 0) HIPCHK(hipMemcpyAsync(ctx->h_in[k].p, src[k], sizeof(float) * n, hipMemcpyHostToDevice, s), "H2D");
+  }
+  const size_t b1 = sizeof(float) * H.u1_cells, b11 = sizeof(float) * std::max<long long>(H.u11_cells, 1);
+  HIPCHK(ctx->h_out[0].ensure(b1), "allocating U1");
+  HIPCHK(ctx->h_out[1].ensure(b11), "allocating U11");
+  // untouched padding keeps the caller's values (INIT_GRIDVARS' -1, alloc.f:162-186)
+  HIPCHK(hipMemcpyAsync(ctx->h_out[0].p, u1, b1, hipMemcpyHostToDevice, s), "H2D");
+  if (H.u11_cells > 0 && u11) HIPCHK(hipMemcpyAsync(ctx->h_out[1].p, u11, b11, hipMemcpyHostToDevice, s), "H2D");
+  double* dmass = nullptr;
+  if (grid_mass) {
+    HIPCHK(ctx->h_mass.ensure(sizeof(double) * H.grids.size()), "allocating masses");
+    dmass = ctx->h_mass.as<double>();
+  }
+  o.stream = (void*)s;
+  st = asohf_hip_density(ctx, geom, n, ctx->h_in[0].as<float>(), ctx->h_in[1].as<float>(),
+                         ctx->h_in[2].as<float>(), ctx->h_in[3].as<float>(), npart_esp,
+                         ctx->h_out[0].as<float>(), ctx->h_out[1].as<float>(), dmass, &o);
+  if (st != ASOHF_OK) return st;
+  HIPCHK(hipMemcpyAsync(u1, ctx->h_out[0].p, b1, hipMemcpyDeviceToHost, s), "D2H");
+  if (H.u11_cells > 0 && u11) HIPCHK(hipMemcpyAsync(u11, ctx->h_out[1].p, b11, hipMemcpyDeviceToHost, s), "D2H");
+  if (grid_mass) HIPCHK(hipMemcpyAsync(grid_mass, dmass, sizeof(double) * H.grids.size(), hipMemcpyDeviceToHost, s), "D2H");
+  HIPCHK(hipStreamSynchronize(s), "host density call");
+  return ASOHF_OK;
+}
+
+asohf_status asohf_hip_density_adjoint(asohf_ctx* ctx, const asohf_geom* geom, int64_t n, const float* rx,
+                                       const float* ry, const float* rz, const float* m, const float* g1,
+                                       const float* g11, float* dm, float* dxp, float* dyp, float* dzp,
+                                       const asohf_opts* opts) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  asohf_opts o = opts ? *opts : default_opts();
+  if (o.mode != ASOHF_MODE_TSC)
+    return fail(ctx, ASOHF_ERR_INVALID, "the adjoint is defined for the TSC mode (table weights are piecewise constant)");
+  asohf_status st = check_geom(ctx, geom, o.mode);
+  if (st != ASOHF_OK) return st;
+  if (n < 0 || (n > 0 && (!rx || !ry || !rz || !m || !dm || !dxp || !dyp || !dzp)) || !g1)
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL array");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  st = build_plan(ctx, geom, o.mode, &o);
+  if (st != ASOHF_OK) return st;
+  Parts A;
+  st = make_parts(ctx, o, n, rx, ry, rz, m, nullptr, A);
+  if (st != ASOHF_OK) return st;
+  hipStream_t s = o.stream ? (hipStream_t)o.stream : ctx->stream;
+  HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
+  if (n > 0) HIPCHK(launch_adjoint(ctx->plan.dev, A, g1, g11, dm, dxp, dyp, dzp, ctx->cus, s), "adjoint kernel");
+  HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
+  HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
+  ctx->stats = asohf_stats{};
+  ctx->stats.n_particles = n;
+  ctx->stats_pending = true;
+  return ASOHF_OK;
+}
+
+asohf_status asohf_hip_membership(asohf_ctx* ctx, const asohf_geom* geom, int64_t n, const float* rx,
+                                  const float* ry, const float* rz, const int32_t* npart_esp, int32_t mode,
+                                  int32_t* out, int64_t cap, int64_t* count) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  asohf_opts o = default_opts();
+  o.mode = mode;
+  asohf_status st = check_geom(ctx, geom, mode);
+  if (st != ASOHF_OK) return st;
+  if (!count || (cap > 0 && !out)) return fail(ctx, ASOHF_ERR_INVALID, "NULL output");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  st = build_plan(ctx, geom, mode, &o);
+  if (st != ASOHF_OK) return st;
+  Parts A;
+  // positions stand in for the masses: the membership pass never reads them
+  st = make_parts(ctx, o, n, rx, ry, rz, rx, npart_esp, A);
+  if (st != ASOHF_OK) return st;
+  HIPCHK(ctx->counters.ensure(sizeof(Counters)), "allocating counters");
+  unsigned long long* c = (unsigned long long*)ctx->counters.p;
+  hipStream_t s = ctx->stream;
+  HIPCHK(launch_fill_u32((unsigned*)c, 0u, 2, s), "clearing");
+  HIPCHK(launch_membership(ctx->plan.dev, A, out, cap, c, ctx->cus, s), "membership kernel");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar, c, sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "D2H");
+  HIPCHK(hipStreamSynchronize(s), "membership");
+  *count = (int64_t)*ctx->hscalar;
+  return ASOHF_OK;
+}
+
+asohf_status asohf_hip_get_tune(const asohf_ctx* ctx, asohf_tune* out) {
+  if (!ctx || !out) return ASOHF_ERR_INVALID;
+  *out = ctx->tune;
+  return ASOHF_OK;
+}
+
+asohf_status asohf_hip_set_tune(asohf_ctx* ctx, const asohf_tune* t) {
+  if (!ctx || !t) return ASOHF_ERR_INVALID;
+  if (!tile_shape_supported(t->tile)) return fail(ctx, ASOHF_ERR_INVALID, "unsupported tile shape");
+  if (t->block < 64 || t->block > 256 || t->block % 64)
+    return fail(ctx, ASOHF_ERR_INVALID, "block must be 64, 128, 192 or 256 threads");
+  if (t->chunk < 64 || t->waves_per_cu < 1 || t->waves_per_cu > 64)
+    return fail(ctx, ASOHF_ERR_INVALID, "chunk >= 64 and 1 <= waves_per_cu <= 64");
+  const bool reshape = memcmp(t->tile, ctx->tune.tile, sizeof(t->tile)) != 0;
+  ctx->tune = *t;
+  if (reshape) ctx->plan.valid = false;
+  return ASOHF_OK;
+}
+
+asohf_status asohf_hip_autotune(asohf_ctx* ctx, const asohf_geom* geom, int64_t n, const float* rx,
+                                const float* ry, const float* rz, const float* m, const int32_t* npart_esp,
+                                float* u1, float* u11, const asohf_opts* opts, asohf_tune* out) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  asohf_opts o = opts ? *opts : default_opts();
+  o.path = ASOHF_PATH_BATCHED;
+  static const int shapes[][3] = {{8, 8, 8}, {16, 8, 8}, {16, 16, 8}, {16, 16, 16}};
+  static const int chunks[] = {2048, 8192};
+  static const int waves[] = {8, 16, 32};
+  asohf_tune best = ctx->tune;
+  double best_ms = 1e30;
+  for (auto& sh : shapes)
+    for (int ch : chunks)
+      for (int wv : waves) {
+        asohf_tune t{{sh[0], sh[1], sh[2]}, 256, ch, wv};
+        asohf_status st = asohf_hip_set_tune(ctx, &t);
+        if (st != ASOHF_OK) return st;
+        double ms = 0;
+        for (int rep = 0; rep < 3; ++rep) {
+          st = asohf_hip_density(ctx, geom, n, rx, ry, rz, m, npart_esp, u1, u11, nullptr, &o);
+          if (st != ASOHF_OK) return st;
+          asohf_stats stt;
+          st = asohf_hip_get_stats(ctx, &stt);
+          if (st != ASOHF_OK) return st;
+          if (rep > 0) ms += stt.ms_total;
+        }
+        if (ms < best_ms) { best_ms = ms; best = t; }
+      }
+  asohf_status st = asohf_hip_set_tune(ctx, &best);
+  if (st != ASOHF_OK) return st;
+  if (out) *out = best;
+  return ASOHF_OK;
+}
+
+asohf_status asohf_hip_get_stats(const asohf_ctx* cctx, asohf_stats* out) {
+  if (!cctx || !out) return ASOHF_ERR_INVALID;
+  asohf_ctx* ctx = const_cast<asohf_ctx*>(cctx);
+  if (ctx->stats_pending) {
+    HIPCHK(hipEventSynchronize(ctx->ev[2]), "waiting for the call");
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[2]), "timing");
+    HIPCHK(hipEventElapsedTime(&b, ctx->ev[0], ctx->ev[1]), "timing");
+    ctx->stats.ms_total = a;
+    ctx->stats.ms_bin = b;
+    ctx->stats.ms_deposit = a - b;
+    ctx->stats_pending = false;
+  }
+  *out = ctx->stats;
+  return ASOHF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+// dev.h -- device helpers of the density build (gfx950).
+//
+// Index expressions are the reference's FP32 expressions evaluated in the
+// same operation order (the library is compiled with -ffp-contract=off and
+// IEEE-correct FP32 division), so every cell and patch index matches the
+// reference bit for bit:
+//   table mode base:  IX = INT((XP-XL)/DX)+1, single periodic wrap  (grids.f:798-813)
+//   table mode patch: strict box with the DX extension, IX = FLOOR((XP-XLFIX)/DXPA)+1,
+//                     stencil half-width N = 2**max(IR-species,0)  (grids.f:871-935)
+//   TSC base:         IX = INT((XP-XL)/DX)+1 clamped to [1,NX]      (grids.f:1180-1192)
+//   TSC patch:        strict BUF=1 box, IX = INT((XP-XL)/DXPA)+1-1  (grids.f:1294-1323)
+#pragma once
+#include "internal.h"
+
+namespace asohf {
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// single periodic wrap of grids.f:808-813 (the TSC wrap of grids.f:1221-1226
+// is the same map on the clamped index range)
+__device__ __forceinline__ int wrap1(int i, int n) {
+  if (i > n) i -= n;
+  if (i < 1) i += n;
+  return i;
+}
+
+// TSC weights of grids.f:1194-1213 with centres c0 = RAD(I-1), c1 = RAD(I), c2 = RAD(I+1)
+__device__ __forceinline__ void tsc_weights(float xp, float c0, float c1, float c2, float d, float w[3]) {
+  float b, t;
+  b = fabsf(xp - c0) / d; t = 1.5f - b; w[0] = 0.5f * (t * t);
+  b = fabsf(xp - c1) / d; w[1] = 0.75f - b * b;
+  b = fabsf(xp - c2) / d; t = 1.5f - b; w[2] = 0.5f * (t * t);
+}
+// derivatives of the three weights with respect to xp
+__device__ __forceinline__ void tsc_dweights(float xp, float c0, float c1, float c2, float d, float dw[3]) {
+  float t;
+  t = xp - c0; dw[0] = -(1.5f - fabsf(t) / d) * (t > 0.f ? 1.f : (t < 0.f ? -1.f : 0.f)) / d;
+  t = xp - c1; dw[1] = -2.0f * t / (d * d);
+  t = xp - c2; dw[2] = -(1.5f - fabsf(t) / d) * (t > 0.f ? 1.f : (t < 0.f ? -1.f : 0.f)) / d;
+}
+
+// MINIRADX(II) = PATCHRX + (FLOAT(II)-1.5)*DXPA  (grids.f:1304-1312)
+__device__ __forceinline__ float minirad(float pr, int ii, float d) { return pr + ((float)ii - 1.5f) * d; }
+
+// INT((XP-XL)/DX)+1 clamped, on the base grid; t is guarded so the
+// conversion is always defined.
+__device__ __forceinline__ int tsc_base_index(float xp, float xl, float d, int n) {
+  float t = (xp - xl) / d;
+  if (!(t < 1.0e9f)) t = 1.0e9f;
+  if (!(t > -1.0e9f)) t = -1.0e9f;
+  return clampi((int)t + 1, 1, n);
+}
+
+__host__ __device__ __forceinline__ uint64_t hash_mix(uint64_t k) {
+  k ^= k >> 33; k *= 0xff51afd7ed558ccdull; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull; k ^= k >> 33;
+  return k;
+}
+__host__ __device__ __forceinline__ uint64_t bucket_key(int level, int bx, int by, int bz) {
+  return ((uint64_t)level << 60) | ((uint64_t)bx << 40) | ((uint64_t)by << 20) | (uint64_t)bz;
+}
+
+// candidate patches of level l whose (inflated) membership box may contain (x,y,z)
+__device__ __forceinline__ int2 bucket_lookup(const Plan& P, int l, float x, float y, float z) {
+  const LevelHash h = P.lh[l];
+  int bx = clampi((int)floorf(fminf(fmaxf((x - h.x0) * h.inv_bs, -1.f), 1.0e6f)), 0, h.nb - 1);
+  int by = clampi((int)floorf(fminf(fmaxf((y - h.x0) * h.inv_bs, -1.f), 1.0e6f)), 0, h.nb - 1);
+  int bz = clampi((int)floorf(fminf(fmaxf((z - h.x0) * h.inv_bs, -1.f), 1.0e6f)), 0, h.nb - 1);
+  const uint64_t key = bucket_key(l, bx, by, bz);
+  unsigned s = (unsigned)hash_mix(key) & P.hmask;
+  // the table is at most half full, so an empty slot ends every probe
+  for (;;) {
+    uint64_t k = P.hkeys[s];
+    if (k == key) return P.hvals[s];
+    if (k == kHashEmpty) return make_int2(0, 0);
+    s = (s + 1) & P.hmask;
+  }
+}
+
+__device__ __forceinline__ int species_of(const Parts& A, long long i) {
+  int s = 0;
+#pragma unroll
+  for (int k = 1; k < kNEsp; ++k) s += (i >= A.sp[k]) ? 1 : 0;
+  return s;
+}
+
+// Table-mode entry: cell index triple (int16 each), kernel level, mass.
+__device__ __forceinline__ int4 pack_table(int ix, int jy, int kz, int irk, float m) {
+  int4 e;
+  e.x = (ix & 0xffff) | (jy << 16);
+  e.y = (kz & 0xffff) | (irk << 16);
+  e.z = __float_as_int(m);
+  e.w = 0;
+  return e;
+}
+__device__ __forceinline__ void unpack_table(int4 e, int& ix, int& jy, int& kz, int& irk, float& m) {
+  ix = (int)(short)(e.x & 0xffff);
+  jy = e.x >> 16;
+  kz = (int)(short)(e.y & 0xffff);
+  irk = e.y >> 16;
+  m = __int_as_float(e.z);
+}
+
+// Tiles touched by the wrapped cells {c-1, c, c+1} along one base axis.
+// Returns the number of distinct tile indices (1..3) in t[].
+__device__ __forceinline__ int base_axis_tiles(int c, int n, int T, int t[3]) {
+  int a = (wrap1(c - 1, n) - 1) / T, b = (wrap1(c, n) - 1) / T, d = (wrap1(c + 1, n) - 1) / T;
+  int k = 0;
+  t[k++] = a;
+  if (b != a) t[k++] = b;
+  if (d != a && d != b) t[k++] = d;
+  return k;
+}
+
+// Enumerates every (grid, tile) a particle deposits into, calling
+//   emit(g, tile, wide, entry)
+// once per touched tile, and member(g, ix, jy, kz) once per (particle, grid)
+// member of the mode's box rule with the reference's cell index of that grid
+// (before any wrap or clipping).  err receives validation flags.
+template <int MODE, class Emit, class Member>
+__device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, float x, float y,
+                                                   float z, float m, int species, Emit&& emit,
+                                                   Member&& member, unsigned& err) {
+  const GridDesc& B = P.grids[0];
+  // ---------------- base grid ----------------
+  int ix, jy, kz;
+  if (MODE == 2) {
+    ix = tsc_base_index(x, P.xl, B.d[0], B.n[0]);
+    jy = tsc_base_index(y, P.xl, B.d[1], B.n[1]);
+    kz = tsc_base_index(z, P.xl, B.d[2], B.n[2]);
+  } else {
+    float tx = (x - P.xl) / B.d[0], ty = (y - P.xl) / B.d[1], tz = (z - P.xl) / B.d[2];
+    // INT() is defined only on a bounded range; beyond it the reference would
+    // index outside U1 anyway
+    if (!(tx > -1.0e9f && tx < 1.0e9f && ty > -1.0e9f && ty < 1.0e9f && tz > -1.0e9f && tz < 1.0e9f)) {
+      err |= kErrWrap;
+      return;
+    }
+    ix = (int)tx + 1; jy = (int)ty + 1; kz = (int)tz + 1;
+    // the single wrap must land in 1..N for all three offsets
+    if (ix < 2 - B.n[0] || ix > 2 * B.n[0] - 1 || jy < 2 - B.n[1] || jy > 2 * B.n[1] - 1 ||
+        kz < 2 - B.n[2] || kz > 2 * B.n[2] - 1) { err |= kErrWrap; return; }
+  }
+  member(0, ix, jy, kz);
+  {
+    int tx[3], ty[3], tz[3];
+    const int nx_ = base_axis_tiles(ix, B.n[0], P.tile[0], tx);
+    const int ny_ = base_axis_tiles(jy, B.n[1], P.tile[1], ty);
+    const int nz_ = base_axis_tiles(kz, B.n[2], P.tile[2], tz);
+    int4 e;
+    if (MODE == 2) e = make_int4(__float_as_int(x), __float_as_int(y), __float_as_int(z), __float_as_int(m));
+    else e = pack_table(ix, jy, kz, 0, m);
+    for (int c = 0; c < nz_; ++c)
+      for (int b = 0; b < ny_; ++b)
+        for (int a = 0; a < nx_; ++a)
+          emit(0, B.tile0 + (tz[c] * B.nt[1] + ty[b]) * B.nt[0] + tx[a], 0, e);
+  }
+  // ---------------- patches, level by level ----------------
+  for (int l = 1; l <= P.nl; ++l) {
+    const int2 se = bucket_lookup(P, l, x, y, z);
+    for (int q = 0; q < se.y; ++q) {
+      const int g = P.hlist[se.x + q];
+      const GridDesc& G = P.grids[g];
+      if (!(G.lo[0] < x && x < G.hi[0] && G.lo[1] < y && y < G.hi[1] && G.lo[2] < z && z < G.hi[2]))
+        continue;
+      int lo[3], hi[3], ci[3];
+      int irk = 0;
+      if (MODE == 2) {
+        ci[0] = (int)((x - G.org[0]) / G.d[0]);
+        ci[1] = (int)((y - G.org[1]) / G.d[1]);
+        ci[2] = (int)((z - G.org[2]) / G.d[2]);
+        member(g, ci[0], ci[1], ci[2]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { lo[a] = max(ci[a] - 1, 1); hi[a] = min(ci[a] + 1, G.n[a]); }
+      } else {
+        ci[0] = (int)floorf((x - G.org[0]) / G.d[0]) + 1;
+        ci[1] = (int)floorf((y - G.org[1]) / G.d[1]) + 1;
+        ci[2] = (int)floorf((z - G.org[2]) / G.d[2]) + 1;
+        member(g, ci[0], ci[1], ci[2]);
+        irk = max(G.level - species, 0);
+        const int nk = 1 << irk;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { lo[a] = max(ci[a] - nk, 1); hi[a] = min(ci[a] + nk, G.n[a]); }
+      }
+      if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) continue;   // cloud misses the patch
+      int4 e;
+      if (MODE == 2) e = make_int4(__float_as_int(x), __float_as_int(y), __float_as_int(z), __float_as_int(m));
+      else e = pack_table(ci[0], ci[1], ci[2], irk, m);
+      const int wide = (MODE == 1 && irk > 0) ? 1 : 0;
+      const int t0x = (lo[0] - 1) / P.tile[0], t1x = (hi[0] - 1) / P.tile[0];
+      const int t0y = (lo[1] - 1) / P.tile[1], t1y = (hi[1] - 1) / P.tile[1];
+      const int t0z = (lo[2] - 1) / P.tile[2], t1z = (hi[2] - 1) / P.tile[2];
+      for (int c = t0z; c <= t1z; ++c)
+        for (int b = t0y; b <= t1y; ++b)
+          for (int a = t0x; a <= t1x; ++a)
+            emit(g, G.tile0 + (c * G.nt[1] + b) * G.nt[0] + a, wide, e);
+    }
+  }
+}
+
+}  // namespace asohf

@@ -1,0 +1,117 @@
+// internal.h -- structures shared by the host planner and the gfx950 kernels
+// of libasohf_hip.  Everything here is plain data: kernels receive these by
+// value or through device pointers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace asohf {
+
+constexpr int kNEsp = 9;       // asohf_parameters.dat: N_ESP
+constexpr int kNLevels = 9;    // asohf_parameters.dat: NLEVELS
+constexpr uint64_t kHashEmpty = ~0ull;
+
+// One grid of the build: the base grid (g = 0) or patch p (g = 1 + p).
+// Box constants are FP32 values computed on the host with the reference's own
+// expressions (grids.f:871-886 for the table mode, grids.f:1294-1302 for TSC),
+// so the device only compares and indexes.
+struct GridDesc {
+  int n[3];          // cells per axis
+  int level;         // IR (0 = base)
+  int nt[3];         // tiles per axis
+  int tile0;         // first tile id
+  float lo[3];       // strict membership box: lo < x < hi
+  float hi[3];
+  float org[3];      // table mode: XLFIX (patch) / XL (base); TSC mode: XL of the BUF box / XL (base)
+  float pr[3];       // PATCHRX/RY/RZ (TSC cell centres MINIRADX)
+  float d[3];        // cell size DXPA/DYPA/DZPA (DX/DY/DZ on the base grid)
+  float den;         // DENBAS of the level, FP32 as the reference forms it
+  int ld1;           // output row stride (elements)
+  int ld2;           // output plane stride (elements)
+  long long off;     // output offset of cell (1,1,1) in u1 (g = 0) or u11
+};
+
+// Per-level spatial hash of patch membership boxes.
+struct LevelHash {
+  float x0, inv_bs;  // bucket b = clamp(floor((x - x0) * inv_bs), 0, nb-1)
+  int nb;
+  int pad;
+};
+
+// Device view of a planned geometry.
+struct Plan {
+  const GridDesc* grids;     // [1 + P]
+  const int* tile_grid;      // [ntiles]
+  const uint64_t* hkeys;     // [hcap]
+  const int2* hvals;         // [hcap]  (start, count) into hlist
+  const int* hlist;          // grid ids
+  const float* radx;         // RADX(0:NX+1) etc.
+  const float* rady;
+  const float* radz;
+  const float* ktab;         // kernel table [(nl+1) * kw], row ir at ir*kw + kmax
+  int nl;
+  int ngrids;
+  int ntiles;
+  int kmax;                  // 2^nl
+  int kw;                    // 2*kmax + 1
+  unsigned hmask;
+  int tile[3];               // tile shape
+  int mode;                  // 1 table, 2 TSC
+  LevelHash lh[kNLevels + 1];
+  float xl;                  // -LADO0/2
+  float fdm;
+};
+
+// Particle inputs of one call.
+struct Parts {
+  const float* x;
+  const float* y;
+  const float* z;
+  const float* m;
+  long long n;               // particles visited (table mode: SUM(NPART_ESP))
+  long long sp[kNEsp + 1];   // species prefix sums (table mode)
+};
+
+// Device counters of one call (one small buffer, cleared by a fill kernel).
+struct Counters {
+  unsigned long long n_pairs;
+  unsigned long long n_entries;
+  unsigned int err;          // bit 0: non-finite position/mass, bit 1: base index outside
+                             // the single-wrap range, bit 2: negative mass
+  unsigned int mmax_bits;    // max particle mass, as FP32 bits (masses >= 0)
+  unsigned int queue_head;   // persistent work queue
+  unsigned int n_items;
+  unsigned int n_split;
+  unsigned int n_items_w;    // second planning pass: items written so far
+  double scale;              // fixed-point scale (power of two)
+  double inv_scale;
+};
+
+// Work item of the persistent deposit: a chunk of one tile's entry list.
+struct Item {
+  int tile;
+  int flags;                 // bit 0: wide (table-mode N >= 2) entries
+  long long begin;           // entry range [begin, end)
+  long long end;
+};
+
+// Arguments of the persistent deposit.
+struct DepArgs {
+  const int4* entries;       // binned entries, grouped by tile
+  const Item* items;         // work queue
+  const int* nitems_tile;    // items per tile
+  const int* split_slot;     // scratch slot of a split tile, -1 otherwise
+  unsigned* done;            // finished items per split tile
+  void* scratch;             // int64 (deterministic) or FP32 (atomic) tile accumulators
+  Counters* C;
+  float* u1;
+  float* u11;
+  long long* tile_mass_i;    // deterministic: exact fixed-point mass per tile
+  double* tile_mass_d;       // atomic: FP64 mass per tile
+  double scale;              // fixed-point scale
+  double inv_scale;
+};
+
+enum : int { kErrNonFinite = 1, kErrWrap = 2, kErrNegMass = 4, kErrIndexRange = 8 };
+
+}  // namespace asohf

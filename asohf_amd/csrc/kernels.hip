@@ -1,0 +1,933 @@
+// kernels.hip -- gfx950 kernels of the density build and their launchers.
+//
+// Batched pipeline (one call):
+//   count_kernel    particles -> (grid, tile) membership counts     [reads x,y,z,m]
+//   scan_*          exclusive scan of the counts -> entry offsets
+//   scatter_kernel  particles -> 16-byte entries grouped by tile    [reads x,y,z,m]
+//   plan_items      tiles -> work items (chunks of <= chunk entries)
+//   deposit_kernel  ONE persistent launch over the item queue: LDS tile
+//                   accumulation, normalised coalesced stores, FP64-exact
+//                   tile mass partials
+//   grid_sums       fixed-order per-grid mass sums
+// Plain form: one launch per grid with global FP32 atomics (plain_*).
+#include <hip/hip_runtime.h>
+
+#include "dev.h"
+#include "launch.h"
+
+namespace asohf {
+
+// ---------------------------------------------------------------------------
+// wave / block reductions
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ unsigned wave_or(unsigned v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// fixed-shape block sum (deterministic for a given blockDim)
+template <class T>
+__device__ T block_sum(T v, T* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  v = wave_sum(v);
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < nw; ++k) r += sh[k];
+    sh[0] = r;
+  }
+  __syncthreads();
+  r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// fills (used instead of hipMemsetAsync so every step can live in a graph)
+// ---------------------------------------------------------------------------
+__global__ void fill_u32_kernel(unsigned* p, unsigned v, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// count / scatter
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__(256) count_kernel(Plan P, Parts A, unsigned* cnt, Counters* C) {
+  unsigned err = 0;
+  unsigned long long pairs = 0;
+  float mmax = 0.f;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
+    if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) { err |= kErrNonFinite; continue; }
+    if (m < 0.f) { err |= kErrNegMass; continue; }
+    mmax = fmaxf(mmax, m);
+    const int s = MODE == 1 ? species_of(A, i) : 0;
+    enumerate_particle<MODE>(
+        P, i, x, y, z, m, s,
+        [&](int, int t, int wide, int4) { atomicAdd(&cnt[2 * t + wide], 1u); },
+        [&](int, int, int, int) { ++pairs; }, err);
+  }
+  pairs = wave_sum(pairs);
+  mmax = wave_max(mmax);
+  err = wave_or(err);
+  if ((threadIdx.x & 63) == 0) {
+    if (pairs) atomicAdd(&C->n_pairs, pairs);
+    atomicMax(&C->mmax_bits, __float_as_uint(mmax));
+    if (err) atomicOr(&C->err, err);
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) scatter_kernel(Plan P, Parts A, unsigned long long* cursor,
+                                                      int4* entries, unsigned long long cap) {
+  unsigned err = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
+    if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m)) || m < 0.f) continue;
+    const int s = MODE == 1 ? species_of(A, i) : 0;
+    enumerate_particle<MODE>(
+        P, i, x, y, z, m, s,
+        [&](int, int t, int wide, int4 e) {
+          const unsigned long long slot = atomicAdd(&cursor[2 * t + wide], 1ull);
+          if (slot < cap) entries[slot] = e;   // the count pass sized the buffer exactly
+        },
+        [&](int, int, int, int) {}, err);
+  }
+}
+
+// membership records {grid, particle, ix, jy, kz}
+template <int MODE>
+__global__ void __launch_bounds__(256) membership_kernel(Plan P, Parts A, int* out, long long cap,
+                                                         unsigned long long* count) {
+  unsigned err = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
+    if (!(isfinite(x) && isfinite(y) && isfinite(z))) continue;
+    const int s = MODE == 1 ? species_of(A, i) : 0;
+    enumerate_particle<MODE>(
+        P, i, x, y, z, m, s, [&](int, int, int, int4) {},
+        [&](int g, int ix, int jy, int kz) {
+          const unsigned long long slot = atomicAdd(count, 1ull);
+          if ((long long)slot < cap) {
+            int* r = out + 5 * slot;
+            r[0] = g; r[1] = (int)i; r[2] = ix; r[3] = jy; r[4] = kz;
+          }
+        },
+        err);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of u32 counts into u64 offsets (3 kernels)
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 4;
+constexpr int kScanTile = kScanThreads * kScanPer;
+
+__device__ unsigned long long block_exclusive_scan(unsigned long long v, unsigned long long* sh,
+                                                   unsigned long long* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned long long inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned long long t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long run = 0;
+    for (int k = 0; k < nw; ++k) { unsigned long long t = sh[k]; sh[k] = run; run += t; }
+    sh[nw] = run;
+  }
+  __syncthreads();
+  const unsigned long long r = sh[w] + inc - v;
+  *total = sh[nw];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_reduce(const unsigned* in, long long n,
+                                                           unsigned long long* bsum) {
+  __shared__ unsigned long long sh[kScanThreads / 64 + 1];
+  const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanPer;
+  unsigned long long s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k)
+    if (base + k < n) s += in[base + k];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_bsums(unsigned long long* bsum, int nb,
+                                                          unsigned long long* total_out) {
+  __shared__ unsigned long long sh[kScanThreads / 64 + 1];
+  unsigned long long carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += kScanThreads) {
+    const int b = b0 + threadIdx.x;
+    const unsigned long long v = b < nb ? bsum[b] : 0ull;
+    unsigned long long tot;
+    const unsigned long long ex = block_exclusive_scan(v, sh, &tot);
+    if (b < nb) bsum[b] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_apply(const unsigned* in, long long n,
+                                                          const unsigned long long* bsum,
+                                                          unsigned long long* start,
+                                                          unsigned long long* cursor) {
+  __shared__ unsigned long long sh[kScanThreads / 64 + 1];
+  const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanPer;
+  unsigned v[kScanPer];
+  unsigned long long s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0u;
+    s += v[k];
+  }
+  unsigned long long tot;
+  unsigned long long ex = block_exclusive_scan(s, sh, &tot) + bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanPer; ++k) {
+    if (base + k < n) {
+      start[base + k] = ex;
+      if (cursor) cursor[base + k] = ex;
+    }
+    ex += v[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// work items
+// ---------------------------------------------------------------------------
+// pass 0: items per tile, split-tile slots and the totals; pass 1: write the items
+__global__ void plan_items(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
+                           int write, Item* items, int* nitems_tile, int* split_slot, Counters* C) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const long long s0 = start[2 * t], s1 = start[2 * t + 1], s2 = start[2 * t + 2];
+  const long long nn = s1 - s0, nw = s2 - s1;
+  const int cn = (int)((nn + chunk - 1) / chunk), cw = (int)((nw + chunk_wide - 1) / chunk_wide);
+  int tot = cn + cw;
+  const int empty = tot == 0;
+  if (empty) tot = 1;   // an empty tile still writes its zeros
+  if (!write) {
+    atomicAdd(&C->n_items, (unsigned)tot);
+    nitems_tile[t] = tot;
+    split_slot[t] = tot > 1 ? (int)atomicAdd(&C->n_split, 1u) : -1;
+    return;
+  }
+  const unsigned base = atomicAdd(&C->n_items_w, (unsigned)tot);
+  int k = 0;
+  if (empty) items[base] = Item{t, 0, s0, s0};
+  for (int c = 0; c < cn; ++c, ++k) {
+    const long long b = s0 + (long long)c * chunk;
+    items[base + k] = Item{t, 0, b, min(b + (long long)chunk, s1)};
+  }
+  for (int c = 0; c < cw; ++c, ++k) {
+    const long long b = s1 + (long long)c * chunk_wide;
+    items[base + k] = Item{t, 1, b, min(b + (long long)chunk_wide, s2)};
+  }
+}
+
+template <class T>
+__global__ void zero_split_scratch(T* scr, long long per, const Counters* C) {
+  const long long n = (long long)C->n_split * per;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    scr[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// the persistent deposit
+// ---------------------------------------------------------------------------
+template <int RED> struct AccT;
+template <> struct AccT<0> { using T = long long; };
+template <> struct AccT<1> { using T = float; };
+
+__device__ __forceinline__ void lds_add(long long* a, long long v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(a), (unsigned long long)v);
+}
+__device__ __forceinline__ void lds_add(float* a, float v) { atomicAdd(a, v); }
+__device__ __forceinline__ void glb_add(long long* a, long long v) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(a), (unsigned long long)v);
+}
+__device__ __forceinline__ void glb_add(float* a, float v) { atomicAdd(a, v); }
+__device__ __forceinline__ long long glb_load(long long* a) {
+  return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float glb_load(float* a) {
+  return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// 27-cell deposit of one particle into the LDS tile.  cx/cy/cz: the three
+// grid cells per axis (already wrapped on the base grid; 0 = outside the
+// patch), w*: the three weights per axis.  Deterministic mode quantises every
+// FP32 term ((m*wx)*wy)*wz to an integer of the fixed-point scale and gives
+// the centre cell the remainder, so each particle deposits exactly Q(m).
+template <int RED, int TX, int TY, int TZ>
+__device__ __forceinline__ void deposit27(typename AccT<RED>::T* acc, float m, const int lx[3],
+                                          const int ly[3], const int lz[3], const float wx[3],
+                                          const float wy[3], const float wz[3], float scale) {
+  if (RED == 1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if ((unsigned)lz[c] >= (unsigned)TZ) continue;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if ((unsigned)ly[b] >= (unsigned)TY) continue;
+        const float mxy[3] = {m * wx[0] * wy[b], m * wx[1] * wy[b], m * wx[2] * wy[b]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if ((unsigned)lx[a] >= (unsigned)TX) continue;
+          lds_add(reinterpret_cast<float*>(acc) + (lx[a] + TX * (ly[b] + TY * lz[c])), mxy[a] * wz[c]);
+        }
+      }
+    }
+  } else {
+    const int qm = __float2int_rn(m * scale);
+    int rest = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (a == 1 && b == 1 && c == 1) continue;
+          const int q = __float2int_rn(((m * wx[a]) * wy[b]) * wz[c] * scale);
+          rest += q;
+          if ((unsigned)lx[a] < (unsigned)TX && (unsigned)ly[b] < (unsigned)TY &&
+              (unsigned)lz[c] < (unsigned)TZ)
+            lds_add(reinterpret_cast<long long*>(acc) + (lx[a] + TX * (ly[b] + TY * lz[c])), (long long)q);
+        }
+      }
+    }
+    if ((unsigned)lx[1] < (unsigned)TX && (unsigned)ly[1] < (unsigned)TY && (unsigned)lz[1] < (unsigned)TZ)
+      lds_add(reinterpret_cast<long long*>(acc) + (lx[1] + TX * (ly[1] + TY * lz[1])), (long long)(qm - rest));
+  }
+}
+
+template <int MODE, int RED, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
+  using Acc = typename AccT<RED>::T;
+  constexpr int T = TX * TY * TZ;
+  __shared__ Acc acc[T];
+  __shared__ unsigned s_item;
+  __shared__ int s_last;
+  __shared__ long long s_redi[16];
+  __shared__ double s_redd[16];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const float scale = (float)D.scale;
+  const float k0[3] = {P.ktab[P.kmax - 1], P.ktab[P.kmax], P.ktab[P.kmax + 1]};
+
+  for (;;) {
+    if (tid == 0) s_item = atomicAdd(&D.C->queue_head, 1u);
+    __syncthreads();
+    const unsigned idx = __builtin_amdgcn_readfirstlane(s_item);
+    const unsigned n_items = D.C->n_items;
+    if (idx >= n_items) break;
+    const Item it = D.items[idx];
+    const int t = __builtin_amdgcn_readfirstlane(it.tile);
+    const int g = __builtin_amdgcn_readfirstlane(P.tile_grid[t]);
+    const GridDesc G = P.grids[g];
+    const int local = t - G.tile0;
+    const int tx = local % G.nt[0], ty = (local / G.nt[0]) % G.nt[1], tz = local / (G.nt[0] * G.nt[1]);
+    const int i0 = tx * TX + 1, j0 = ty * TY + 1, k0c = tz * TZ + 1;
+    const int ex = min(TX, G.n[0] - i0 + 1), ey = min(TY, G.n[1] - j0 + 1), ez = min(TZ, G.n[2] - k0c + 1);
+
+    for (int c = tid; c < T; c += nthr) acc[c] = 0;
+    __syncthreads();
+
+    if (!(it.flags & 1)) {
+      // ---- narrow entries: one lane per entry, 27 cells ----
+      for (long long e = it.begin + tid; e < it.end; e += nthr) {
+        const int4 v = D.entries[e];
+        int ci[3];
+        float wx[3], wy[3], wz[3], m;
+        if (MODE == 2) {
+          const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
+          m = __int_as_float(v.w);
+          float* w[3] = {wx, wy, wz};
+          if (g == 0) {
+            const float* rad[3] = {P.radx, P.rady, P.radz};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.n[a]);
+              tsc_weights(p[a], rad[a][ci[a] - 1], rad[a][ci[a]], rad[a][ci[a] + 1], G.d[a], w[a]);
+            }
+          } else {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              ci[a] = (int)((p[a] - G.org[a]) / G.d[a]);
+              tsc_weights(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
+                          minirad(G.pr[a], ci[a] + 1, G.d[a]), G.d[a], w[a]);
+            }
+          }
+        } else {
+          int irk;
+          unpack_table(v, ci[0], ci[1], ci[2], irk, m);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) { wx[k] = k0[k]; wy[k] = k0[k]; wz[k] = k0[k]; }
+        }
+        int lx[3], ly[3], lz[3];
+        const int org[3] = {i0, j0, k0c};
+        int* l[3] = {lx, ly, lz};
+        const int ext[3] = {ex, ey, ez};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            int c = ci[a] - 1 + k;
+            bool ok = true;
+            if (g == 0) c = wrap1(c, G.n[a]);
+            else ok = c >= 1 && c <= G.n[a];
+            int loc = c - org[a];
+            l[a][k] = (ok && loc >= 0 && loc < ext[a]) ? loc : 0x40000000;
+          }
+        }
+        deposit27<RED, TX, TY, TZ>(acc, m, lx, ly, lz, wx, wy, wz, scale);
+      }
+    } else if (MODE == 1) {
+      // ---- wide table-mode entries: one wave per entry, lanes over cells ----
+      const int wave = tid >> 6, lane = tid & 63, nwaves = nthr >> 6;
+      for (long long e = it.begin + wave; e < it.end; e += nwaves) {
+        int ci[3], irk;
+        float m;
+        unpack_table(D.entries[e], ci[0], ci[1], ci[2], irk, m);
+        const int nk = 1 << irk;
+        const float* K = P.ktab + irk * P.kw + P.kmax;
+        const int xa = max(max(ci[0] - nk, 1), i0), xb = min(min(ci[0] + nk, G.n[0]), i0 + ex - 1);
+        const int ya = max(max(ci[1] - nk, 1), j0), yb = min(min(ci[1] + nk, G.n[1]), j0 + ey - 1);
+        const int za = max(max(ci[2] - nk, 1), k0c), zb = min(min(ci[2] + nk, G.n[2]), k0c + ez - 1);
+        if (xb < xa || yb < ya || zb < za) continue;
+        const int bx = xb - xa + 1, by = yb - ya + 1, total = bx * by * (zb - za + 1);
+        for (int c = lane; c < total; c += 64) {
+          const int a = c % bx, r = c / bx, b = r % by, cc = r / by;
+          const int ii = xa + a, jj = ya + b, kk = za + cc;
+          const float term = ((m * K[ii - ci[0]]) * K[jj - ci[1]]) * K[kk - ci[2]];
+          Acc* dst = acc + ((ii - i0) + TX * ((jj - j0) + TY * (kk - k0c)));
+          if (RED == 0) lds_add(reinterpret_cast<long long*>(dst), (long long)__float2int_rn(term * scale));
+          else lds_add(reinterpret_cast<float*>(dst), term);
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- merge the chunks of a split tile; the last one writes ----
+    const int nit = D.nitems_tile[t];
+    bool writer = true;
+    if (nit > 1) {
+      Acc* scr = reinterpret_cast<Acc*>(D.scratch) + (long long)D.split_slot[t] * T;
+      for (int c = tid; c < T; c += nthr) {
+        const Acc v = acc[c];
+        if (v != 0) glb_add(scr + c, v);
+      }
+      __threadfence();
+      __syncthreads();
+      if (tid == 0) {
+        const unsigned prev = __hip_atomic_fetch_add(D.done + t, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = prev == (unsigned)(nit - 1);
+      }
+      __syncthreads();
+      writer = s_last;
+      if (writer) {
+        for (int c = tid; c < T; c += nthr) acc[c] = glb_load(scr + c);
+        __syncthreads();
+      }
+    }
+    if (writer) {
+      float* out = (g == 0 ? D.u1 : D.u11) + G.off;
+      long long msi = 0;
+      double msd = 0.0;
+      const double inv_scale = D.inv_scale;
+      for (int c = tid; c < T; c += nthr) {
+        const int a = c % TX, b = (c / TX) % TY, cz = c / (TX * TY);
+        if (a >= ex || b >= ey || cz >= ez) continue;
+        const Acc v = acc[c];
+        if (RED == 0) msi += (long long)v;
+        else msd += (double)v;
+        float f = RED == 0 ? (float)((double)v * inv_scale) : (float)v;
+        f = f / G.den;
+        f = f / P.fdm;
+        out[(long long)(i0 - 1 + a) + (long long)G.ld1 * (j0 - 1 + b) + (long long)G.ld2 * (k0c - 1 + cz)] = f;
+      }
+      if (RED == 0) {
+        msi = block_sum(msi, s_redi);
+        if (tid == 0) D.tile_mass_i[t] = msi;
+      } else {
+        msd = block_sum(msd, s_redd);
+        if (tid == 0) D.tile_mass_d[t] = msd;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// per-grid mass in a fixed order; FP64 (int64-exact in deterministic mode)
+template <int RED>
+__global__ void __launch_bounds__(256) grid_sums(Plan P, DepArgs D, double* grid_mass) {
+  __shared__ long long shi[16];
+  __shared__ double shd[16];
+  const int g = blockIdx.x;
+  const GridDesc& G = P.grids[g];
+  const int nt = G.level == 0 && g != 0 ? 0 : G.nt[0] * G.nt[1] * G.nt[2];
+  if (RED == 0) {
+    long long s = 0;
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) s += D.tile_mass_i[G.tile0 + k];
+    s = block_sum(s, shi);
+    if (threadIdx.x == 0) grid_mass[g] = (double)s * D.inv_scale;
+  } else {
+    double s = 0;
+    for (int k = threadIdx.x; k < nt; k += blockDim.x) s += D.tile_mass_d[G.tile0 + k];
+    s = block_sum(s, shd);
+    if (threadIdx.x == 0) grid_mass[g] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of the TSC mode: one thread per particle, gathers the upstream
+// gradient at every stencil cell of every grid the particle belongs to
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const float* g1, const float* g11,
+                                                      float* dm, float* dxo, float* dyo, float* dzo) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float p[3] = {A.x[i], A.y[i], A.z[i]};
+    const float m = A.m[i];
+    float am = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+    // base grid
+    {
+      const GridDesc& G = P.grids[0];
+      const float* rad[3] = {P.radx, P.rady, P.radz};
+      int ci[3];
+      float w[3][3], dw[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.n[a]);
+        const float c0 = rad[a][ci[a] - 1], c1 = rad[a][ci[a]], c2 = rad[a][ci[a] + 1];
+        tsc_weights(p[a], c0, c1, c2, G.d[a], w[a]);
+        tsc_dweights(p[a], c0, c1, c2, G.d[a], dw[a]);
+      }
+      const float sc = 1.0f / (G.den * P.fdm);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const long long kz = wrap1(ci[2] - 1 + c, G.n[2]) - 1;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const long long jy = wrap1(ci[1] - 1 + b, G.n[1]) - 1;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const long long ix = wrap1(ci[0] - 1 + a, G.n[0]) - 1;
+            const float gv = g1[G.off + ix + G.ld1 * jy + G.ld2 * kz] * sc;
+            am += gv * w[0][a] * w[1][b] * w[2][c];
+            ax += gv * dw[0][a] * w[1][b] * w[2][c];
+            ay += gv * w[0][a] * dw[1][b] * w[2][c];
+            az += gv * w[0][a] * w[1][b] * dw[2][c];
+          }
+        }
+      }
+    }
+    for (int l = 1; l <= P.nl; ++l) {
+      const int2 se = bucket_lookup(P, l, p[0], p[1], p[2]);
+      for (int q = 0; q < se.y; ++q) {
+        const int g = P.hlist[se.x + q];
+        const GridDesc& G = P.grids[g];
+        if (!(G.lo[0] < p[0] && p[0] < G.hi[0] && G.lo[1] < p[1] && p[1] < G.hi[1] && G.lo[2] < p[2] &&
+              p[2] < G.hi[2]))
+          continue;
+        int ci[3];
+        float w[3][3], dw[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          ci[a] = (int)((p[a] - G.org[a]) / G.d[a]);
+          const float c0 = minirad(G.pr[a], ci[a] - 1, G.d[a]), c1 = minirad(G.pr[a], ci[a], G.d[a]),
+                      c2 = minirad(G.pr[a], ci[a] + 1, G.d[a]);
+          tsc_weights(p[a], c0, c1, c2, G.d[a], w[a]);
+          tsc_dweights(p[a], c0, c1, c2, G.d[a], dw[a]);
+        }
+        const float sc = 1.0f / (G.den * P.fdm);
+        const float* gp = g11 + G.off;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const int kz = ci[2] - 1 + c;
+          if (kz < 1 || kz > G.n[2]) continue;
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            const int jy = ci[1] - 1 + b;
+            if (jy < 1 || jy > G.n[1]) continue;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const int ix = ci[0] - 1 + a;
+              if (ix < 1 || ix > G.n[0]) continue;
+              const float gv = gp[(long long)(ix - 1) + (long long)G.ld1 * (jy - 1) + (long long)G.ld2 * (kz - 1)] * sc;
+              am += gv * w[0][a] * w[1][b] * w[2][c];
+              ax += gv * dw[0][a] * w[1][b] * w[2][c];
+              ay += gv * w[0][a] * dw[1][b] * w[2][c];
+              az += gv * w[0][a] * w[1][b] * dw[2][c];
+            }
+          }
+        }
+      }
+    }
+    dm[i] = am;
+    dxo[i] = m * ax;
+    dyo[i] = m * ay;
+    dzo[i] = m * az;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plain form: one launch per grid, global FP32 atomics into the outputs
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) plain_zero(Plan P, float* u1, float* u11) {
+  const GridDesc& G = P.grids[blockIdx.x];
+  if (G.level == 0 && blockIdx.x != 0) return;
+  float* out = (blockIdx.x == 0 ? u1 : u11) + G.off;
+  const long long nc = (long long)G.n[0] * G.n[1] * G.n[2];
+  for (long long c = threadIdx.x; c < nc; c += blockDim.x) {
+    const long long a = c % G.n[0], r = c / G.n[0], b = r % G.n[1], cz = r / G.n[1];
+    out[a + G.ld1 * b + G.ld2 * cz] = 0.f;
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) plain_base(Plan P, Parts A, float* u1) {
+  const GridDesc& G = P.grids[0];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float p[3] = {A.x[i], A.y[i], A.z[i]};
+    const float m = A.m[i];
+    int ci[3];
+    float w[3][3];
+    if (MODE == 2) {
+      const float* rad[3] = {P.radx, P.rady, P.radz};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.n[a]);
+        tsc_weights(p[a], rad[a][ci[a] - 1], rad[a][ci[a]], rad[a][ci[a] + 1], G.d[a], w[a]);
+      }
+    } else {
+      bool ok = true;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float t = (p[a] - P.xl) / G.d[a];
+        ok = ok && t > -1.0e9f && t < 1.0e9f;
+        ci[a] = ok ? (int)t + 1 : 1;
+        ok = ok && ci[a] >= 2 - G.n[a] && ci[a] <= 2 * G.n[a] - 1;
+        w[a][0] = P.ktab[P.kmax - 1]; w[a][1] = P.ktab[P.kmax]; w[a][2] = P.ktab[P.kmax + 1];
+      }
+      if (!ok) continue;   // rejected by the count pass of the same call
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long kz = wrap1(ci[2] - 1 + c, G.n[2]) - 1;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const long long jy = wrap1(ci[1] - 1 + b, G.n[1]) - 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const long long ix = wrap1(ci[0] - 1 + a, G.n[0]) - 1;
+          atomicAdd(u1 + G.off + ix + G.ld1 * jy + G.ld2 * kz, m * w[0][a] * w[1][b] * w[2][c]);
+        }
+      }
+    }
+  }
+}
+
+// input validation of the plain form (same flags as the count pass)
+template <int MODE>
+__global__ void __launch_bounds__(256) plain_validate(Plan P, Parts A, Counters* C) {
+  unsigned err = 0;
+  const GridDesc& B = P.grids[0];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float p[3] = {A.x[i], A.y[i], A.z[i]};
+    const float m = A.m[i];
+    if (!(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]) && isfinite(m))) { err |= kErrNonFinite; continue; }
+    if (m < 0.f) err |= kErrNegMass;
+    if (MODE == 1) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float t = (p[a] - P.xl) / B.d[a];
+        if (!(t > -1.0e9f && t < 1.0e9f)) { err |= kErrWrap; break; }
+        const int c = (int)t + 1;
+        if (c < 2 - B.n[a] || c > 2 * B.n[a] - 1) err |= kErrWrap;
+      }
+    }
+  }
+  err = wave_or(err);
+  if ((threadIdx.x & 63) == 0 && err) atomicOr(&C->err, err);
+}
+
+// level membership flags for the plain form's per-level compaction: flag = 1
+// when the particle lies in some level-l box (TSC: the PLEV rule of
+// grids.f:1114-1157; table mode: the DX-extended boxes of grids.f:871-909)
+__global__ void __launch_bounds__(256) plain_level_flags(Plan P, Parts A, int l, unsigned* flag) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float x = A.x[i], y = A.y[i], z = A.z[i];
+    unsigned f = 0;
+    const int2 se = bucket_lookup(P, l, x, y, z);
+    for (int q = 0; q < se.y && !f; ++q) {
+      const GridDesc& G = P.grids[P.hlist[se.x + q]];
+      if (G.lo[0] < x && x < G.hi[0] && G.lo[1] < y && y < G.hi[1] && G.lo[2] < z && z < G.hi[2]) f = 1;
+    }
+    flag[i] = f;
+  }
+}
+
+__global__ void plain_compact(const unsigned* flag, const unsigned long long* start, long long n,
+                              long long* idx) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    if (flag[i]) idx[start[i]] = i;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) plain_patch(Plan P, Parts A, int g, const long long* idx,
+                                                   const unsigned long long* nidx_p, float* u11) {
+  const GridDesc& G = P.grids[g];
+  const long long nidx = (long long)*nidx_p;
+  float* out = u11 + G.off;
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nidx;
+       k += (long long)gridDim.x * blockDim.x) {
+    const long long i = idx[k];
+    const float p[3] = {A.x[i], A.y[i], A.z[i]};
+    if (!(G.lo[0] < p[0] && p[0] < G.hi[0] && G.lo[1] < p[1] && p[1] < G.hi[1] && G.lo[2] < p[2] && p[2] < G.hi[2]))
+      continue;
+    const float m = A.m[i];
+    if (MODE == 2) {
+      int ci[3];
+      float w[3][3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        ci[a] = (int)((p[a] - G.org[a]) / G.d[a]);
+        tsc_weights(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
+                    minirad(G.pr[a], ci[a] + 1, G.d[a]), G.d[a], w[a]);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int kz = ci[2] - 1 + c;
+        if (kz < 1 || kz > G.n[2]) continue;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const int jy = ci[1] - 1 + b;
+          if (jy < 1 || jy > G.n[1]) continue;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const int ix = ci[0] - 1 + a;
+            if (ix < 1 || ix > G.n[0]) continue;
+            atomicAdd(out + (ix - 1) + (long long)G.ld1 * (jy - 1) + (long long)G.ld2 * (kz - 1),
+                      m * w[0][a] * w[1][b] * w[2][c]);
+          }
+        }
+      }
+    } else {
+      const int s = species_of(A, i);
+      const int irk = max(G.level - s, 0), nk = 1 << irk;
+      const float* K = P.ktab + irk * P.kw + P.kmax;
+      int ci[3], lo[3], hi[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        ci[a] = (int)floorf((p[a] - G.org[a]) / G.d[a]) + 1;
+        lo[a] = max(ci[a] - nk, 1);
+        hi[a] = min(ci[a] + nk, G.n[a]);
+      }
+      for (int kk = lo[2]; kk <= hi[2]; ++kk)
+        for (int jj = lo[1]; jj <= hi[1]; ++jj)
+          for (int ii = lo[0]; ii <= hi[0]; ++ii)
+            atomicAdd(out + (ii - 1) + (long long)G.ld1 * (jj - 1) + (long long)G.ld2 * (kk - 1),
+                      m * K[ii - ci[0]] * K[jj - ci[1]] * K[kk - ci[2]]);
+    }
+  }
+}
+
+// normalise every grid (/DENBAS, then /FDM) and sum its mass in FP64
+__global__ void __launch_bounds__(256) plain_finish(Plan P, float* u1, float* u11, double* grid_mass) {
+  __shared__ double sh[16];
+  const int g = blockIdx.x;
+  const GridDesc& G = P.grids[g];
+  if (G.level == 0 && g != 0) {
+    if (threadIdx.x == 0 && grid_mass) grid_mass[g] = 0.0;
+    return;
+  }
+  float* out = (g == 0 ? u1 : u11) + G.off;
+  const long long nc = (long long)G.n[0] * G.n[1] * G.n[2];
+  double s = 0.0;
+  for (long long c = threadIdx.x; c < nc; c += blockDim.x) {
+    const long long a = c % G.n[0], r = c / G.n[0], b = r % G.n[1], cz = r / G.n[1];
+    float* q = out + a + G.ld1 * b + G.ld2 * cz;
+    const float v = *q;
+    s += v;
+    *q = (v / G.den) / P.fdm;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0 && grid_mass) grid_mass[g] = s;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static int grid_for(long long n, int block, int cap) {
+  long long b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+hipError_t launch_fill_u32(unsigned* p, unsigned v, long long n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  fill_u32_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* C, int cus, hipStream_t s) {
+  const int grid = grid_for(A.n, 256, cus * 16);
+  if (P.mode == 2) count_kernel<2><<<grid, 256, 0, s>>>(P, A, cnt, C);
+  else count_kernel<1><<<grid, 256, 0, s>>>(P, A, cnt, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(const Plan& P, const Parts& A, unsigned long long* cursor, int4* entries,
+                          unsigned long long cap, int cus, hipStream_t s) {
+  const int grid = grid_for(A.n, 256, cus * 16);
+  if (P.mode == 2) scatter_kernel<2><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap);
+  else scatter_kernel<1><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_membership(const Plan& P, const Parts& A, int* out, long long cap,
+                             unsigned long long* count, int cus, hipStream_t s) {
+  const int grid = grid_for(A.n, 256, cus * 16);
+  if (P.mode == 2) membership_kernel<2><<<grid, 256, 0, s>>>(P, A, out, cap, count);
+  else membership_kernel<1><<<grid, 256, 0, s>>>(P, A, out, cap, count);
+  return hipGetLastError();
+}
+
+long long scan_scratch_elems(long long n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+hipError_t launch_scan(const unsigned* in, long long n, unsigned long long* bsum, unsigned long long* start,
+                       unsigned long long* cursor, hipStream_t s) {
+  const int nb = (int)((n + kScanTile - 1) / kScanTile);
+  if (nb > 0) scan_reduce<<<nb, kScanThreads, 0, s>>>(in, n, bsum);
+  scan_bsums<<<1, kScanThreads, 0, s>>>(bsum, nb, start + n);
+  if (nb > 0) scan_apply<<<nb, kScanThreads, 0, s>>>(in, n, bsum, start, cursor);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_items(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
+                             int write, Item* items, int* nitems_tile, int* split_slot, Counters* C,
+                             hipStream_t s) {
+  plan_items<<<grid_for(ntiles, 256, 1 << 30), 256, 0, s>>>(start, ntiles, chunk, chunk_wide, write, items,
+                                                           nitems_tile, split_slot, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_split(void* scr, int reduction, long long per, long long cap_elems,
+                             const Counters* C, hipStream_t s) {
+  const int grid = grid_for(cap_elems, 256, 8192);
+  if (reduction == 0) zero_split_scratch<long long><<<grid, 256, 0, s>>>((long long*)scr, per, C);
+  else zero_split_scratch<float><<<grid, 256, 0, s>>>((float*)scr, per, C);
+  return hipGetLastError();
+}
+
+template <int MODE, int RED, int TX, int TY, int TZ>
+static hipError_t dep(const Plan& P, const DepArgs& D, int grid, int block, hipStream_t s) {
+  deposit_kernel<MODE, RED, TX, TY, TZ><<<grid, block, 0, s>>>(P, D);
+  return hipGetLastError();
+}
+
+template <int MODE, int RED>
+static hipError_t dep_shape(const Plan& P, const DepArgs& D, int grid, int block, hipStream_t s) {
+  const int* t = P.tile;
+  if (t[0] == 8 && t[1] == 8 && t[2] == 8) return dep<MODE, RED, 8, 8, 8>(P, D, grid, block, s);
+  if (t[0] == 16 && t[1] == 8 && t[2] == 8) return dep<MODE, RED, 16, 8, 8>(P, D, grid, block, s);
+  if (t[0] == 16 && t[1] == 16 && t[2] == 8) return dep<MODE, RED, 16, 16, 8>(P, D, grid, block, s);
+  if (t[0] == 16 && t[1] == 16 && t[2] == 16) return dep<MODE, RED, 16, 16, 16>(P, D, grid, block, s);
+  return hipErrorInvalidValue;
+}
+
+bool tile_shape_supported(const int t[3]) {
+  return (t[0] == 8 && t[1] == 8 && t[2] == 8) || (t[0] == 16 && t[1] == 8 && t[2] == 8) ||
+         (t[0] == 16 && t[1] == 16 && t[2] == 8) || (t[0] == 16 && t[1] == 16 && t[2] == 16);
+}
+
+hipError_t launch_deposit(const Plan& P, const DepArgs& D, int reduction, int grid, int block, hipStream_t s) {
+  if (P.mode == 2) return reduction == 0 ? dep_shape<2, 0>(P, D, grid, block, s) : dep_shape<2, 1>(P, D, grid, block, s);
+  return reduction == 0 ? dep_shape<1, 0>(P, D, grid, block, s) : dep_shape<1, 1>(P, D, grid, block, s);
+}
+
+hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, double* grid_mass, hipStream_t s) {
+  if (P.ngrids <= 0) return hipSuccess;
+  if (reduction == 0) grid_sums<0><<<P.ngrids, 256, 0, s>>>(P, D, grid_mass);
+  else grid_sums<1><<<P.ngrids, 256, 0, s>>>(P, D, grid_mass);
+  return hipGetLastError();
+}
+
+hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
+                          float* dx, float* dy, float* dz, int cus, hipStream_t s) {
+  adjoint_gather<<<grid_for(A.n, 256, cus * 16), 256, 0, s>>>(P, A, g1, g11, dm, dx, dy, dz);
+  return hipGetLastError();
+}
+
+hipError_t launch_plain_zero(const Plan& P, float* u1, float* u11, hipStream_t s) {
+  plain_zero<<<P.ngrids, 256, 0, s>>>(P, u1, u11);
+  return hipGetLastError();
+}
+hipError_t launch_plain_base(const Plan& P, const Parts& A, float* u1, int cus, hipStream_t s) {
+  const int grid = grid_for(A.n, 256, cus * 16);
+  if (P.mode == 2) plain_base<2><<<grid, 256, 0, s>>>(P, A, u1);
+  else plain_base<1><<<grid, 256, 0, s>>>(P, A, u1);
+  return hipGetLastError();
+}
+hipError_t launch_plain_validate(const Plan& P, const Parts& A, Counters* C, int cus, hipStream_t s) {
+  const int grid = grid_for(A.n, 256, cus * 16);
+  if (P.mode == 2) plain_validate<2><<<grid, 256, 0, s>>>(P, A, C);
+  else plain_validate<1><<<grid, 256, 0, s>>>(P, A, C);
+  return hipGetLastError();
+}
+hipError_t launch_plain_flags(const Plan& P, const Parts& A, int l, unsigned* flag, int cus, hipStream_t s) {
+  plain_level_flags<<<grid_for(A.n, 256, cus * 16), 256, 0, s>>>(P, A, l, flag);
+  return hipGetLastError();
+}
+hipError_t launch_plain_compact(const unsigned* flag, const unsigned long long* start, long long n,
+                                long long* idx, int cus, hipStream_t s) {
+  plain_compact<<<grid_for(n, 256, cus * 16), 256, 0, s>>>(flag, start, n, idx);
+  return hipGetLastError();
+}
+hipError_t launch_plain_patch(const Plan& P, const Parts& A, int g, const long long* idx,
+                              const unsigned long long* nidx, long long nmax, float* u11, int cus,
+                              hipStream_t s) {
+  if (nmax <= 0) return hipSuccess;
+  const int grid = grid_for(nmax, 256, cus * 8);
+  if (P.mode == 2) plain_patch<2><<<grid, 256, 0, s>>>(P, A, g, idx, nidx, u11);
+  else plain_patch<1><<<grid, 256, 0, s>>>(P, A, g, idx, nidx, u11);
+  return hipGetLastError();
+}
+hipError_t launch_plain_finish(const Plan& P, float* u1, float* u11, double* grid_mass, hipStream_t s) {
+  plain_finish<<<P.ngrids, 256, 0, s>>>(P, u1, u11, grid_mass);
+  return hipGetLastError();
+}
+
+}  // namespace asohf

@@ -1,0 +1,39 @@
+// launch.h -- host launchers of the kernels in kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace asohf {
+
+hipError_t launch_fill_u32(unsigned* p, unsigned v, long long n, hipStream_t s);
+hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* C, int cus, hipStream_t s);
+hipError_t launch_scatter(const Plan& P, const Parts& A, unsigned long long* cursor, int4* entries,
+                          unsigned long long cap, int cus, hipStream_t s);
+hipError_t launch_membership(const Plan& P, const Parts& A, int* out, long long cap,
+                             unsigned long long* count, int cus, hipStream_t s);
+long long scan_scratch_elems(long long n);
+hipError_t launch_scan(const unsigned* in, long long n, unsigned long long* bsum, unsigned long long* start,
+                       unsigned long long* cursor, hipStream_t s);
+hipError_t launch_plan_items(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
+                             int write, Item* items, int* nitems_tile, int* split_slot, Counters* C,
+                             hipStream_t s);
+hipError_t launch_zero_split(void* scr, int reduction, long long per, long long cap_elems,
+                             const Counters* C, hipStream_t s);
+bool tile_shape_supported(const int t[3]);
+hipError_t launch_deposit(const Plan& P, const DepArgs& D, int reduction, int grid, int block, hipStream_t s);
+hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, double* grid_mass, hipStream_t s);
+hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
+                          float* dx, float* dy, float* dz, int cus, hipStream_t s);
+hipError_t launch_plain_zero(const Plan& P, float* u1, float* u11, hipStream_t s);
+hipError_t launch_plain_base(const Plan& P, const Parts& A, float* u1, int cus, hipStream_t s);
+hipError_t launch_plain_flags(const Plan& P, const Parts& A, int l, unsigned* flag, int cus, hipStream_t s);
+hipError_t launch_plain_compact(const unsigned* flag, const unsigned long long* start, long long n,
+                                long long* idx, int cus, hipStream_t s);
+hipError_t launch_plain_validate(const Plan& P, const Parts& A, Counters* C, int cus, hipStream_t s);
+hipError_t launch_plain_patch(const Plan& P, const Parts& A, int g, const long long* idx,
+                              const unsigned long long* nidx, long long nmax, float* u11, int cus,
+                              hipStream_t s);
+hipError_t launch_plain_finish(const Plan& P, float* u1, float* u11, double* grid_mass, hipStream_t s);
+
+}  // namespace asohf

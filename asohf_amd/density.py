@@ -1,0 +1,281 @@
+"""Python face of libasohf_hip: the density build on device tensors.
+
+This mirrors the reference's DENSITY call (grids.f:710-713, called at
+asohf.f:467-470): the same inputs (particle positions and masses in reference
+order, NPART_ESP, the patch hierarchy, LADO0, the COMMON /ESPACIADO/ and
+/BACK/ constants) and the same outputs (U1 on the base grid, U11 on every
+patch), plus the FP64 deposited mass per grid.  Status codes of the C ABI
+become AsohfError exceptions, the analogue of the reference's STOP 1 on bad
+input.
+
+PyTorch supplies device memory and streams; every computation runs in the HIP
+library.  There is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+import torch
+
+from . import _lib
+from .geometry import N_ESP, Geometry
+
+_MODES = {"table": _lib.MODE_TABLE, "A": _lib.MODE_TABLE, "tsc": _lib.MODE_TSC, "B": _lib.MODE_TSC}
+_REDS = {"deterministic": _lib.REDUCE_DETERMINISTIC, "atomic": _lib.REDUCE_ATOMIC}
+_PATHS = {"batched": _lib.PATH_BATCHED, "plain": _lib.PATH_PLAIN}
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _np_ptr(a, typ):
+    return a.ctypes.data_as(C.POINTER(typ))
+
+
+class _GeomArgs:
+    """Keeps the host arrays behind a Geom struct alive for one call."""
+
+    def __init__(self, g: Geometry, pass_rad: bool = True):
+        self.keep = [g.npatch, g.pnx, g.pny, g.pnz, g.prx, g.pry, g.prz, g.radx, g.rady, g.radz]
+        f, i = C.c_float, C.c_int32
+        self.s = _lib.Geom(g.nx, g.ny, g.nz, g.nl, _np_ptr(g.npatch, i), _np_ptr(g.pnx, i),
+                           _np_ptr(g.pny, i), _np_ptr(g.pnz, i), _np_ptr(g.prx, f), _np_ptr(g.pry, f),
+                           _np_ptr(g.prz, f), float(g.dx), float(g.dy), float(g.dz), float(g.lado0),
+                           float(g.rote), float(g.rete), float(g.fdm), int(g.interp_degree),
+                           _np_ptr(g.radx, f) if pass_rad else None,
+                           _np_ptr(g.rady, f) if pass_rad else None,
+                           _np_ptr(g.radz, f) if pass_rad else None)
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+@dataclasses.dataclass
+class DensityResult:
+    u1: torch.Tensor            # base field, [nz, ly, lx] (lx, ly = leading dims)
+    u11: torch.Tensor           # patch fields, packed (or padded) 1-D
+    grid_mass: torch.Tensor | None   # FP64 [1+P], deposited mass per grid
+
+
+def _check_particles(*ts):
+    n = None
+    for t in ts:
+        if t is None:
+            continue
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.dim() == 1):
+            raise ValueError("particle arrays must be contiguous 1-D float32 tensors on a HIP device")
+        if n is None:
+            n = t.numel()
+        elif t.numel() != n:
+            raise ValueError("particle arrays differ in length")
+    return n or 0
+
+
+class DensityBuilder:
+    """One library context (one device, one stream) and its workspaces."""
+
+    def __init__(self, device: int | None = None):
+        L = _lib.load()
+        if device is None:
+            device = torch.cuda.current_device()
+        self.device = int(device)
+        h = C.c_void_p()
+        _lib.check(None, L.asohf_hip_create(C.byref(h), self.device))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().asohf_hip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ options
+    def _opts(self, mode, reduction="deterministic", path="batched", u1_ld=None, u11_ld=None):
+        o = _lib.Opts()
+        o.mode = _MODES[mode]
+        o.reduction = _REDS[reduction]
+        o.path = _PATHS[path]
+        o.use_graph = 0
+        if u1_ld:
+            o.u1_ld[0], o.u1_ld[1] = u1_ld
+        if u11_ld:
+            o.u11_ld[0], o.u11_ld[1], o.u11_ld[2] = u11_ld
+        o.stream = C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        return o
+
+    @staticmethod
+    def out_shapes(geom: Geometry, u1_ld=None, u11_ld=None):
+        lx, ly = u1_ld if u1_ld else (geom.nx, geom.ny)
+        n11 = (geom.npatch_total * u11_ld[0] * u11_ld[1] * u11_ld[2] if u11_ld
+               else int(geom.patch_cells().sum()))
+        return (geom.nz, ly, lx), max(n11, 1)
+
+    # ------------------------------------------------------------------ forward
+    def density(self, geom: Geometry, x, y, z, m, npart_esp=None, *, mode="table",
+                reduction="deterministic", path="batched", u1=None, u11=None, want_mass=True,
+                u1_ld=None, u11_ld=None) -> DensityResult:
+        """U1/U11 of DENSITY (mode "table") or INTERPOLATE_DENSITY_TSC (mode
+        "tsc"), both divided by FDM as RENORM_DENSITY does."""
+        n = _check_particles(x, y, z, m)
+        dev = x.device
+        s1, n11 = self.out_shapes(geom, u1_ld, u11_ld)
+        if u1 is None:
+            u1 = torch.empty(s1, dtype=torch.float32, device=dev)
+        if u11 is None:
+            u11 = torch.empty(n11, dtype=torch.float32, device=dev)
+        gm = torch.empty(1 + geom.npatch_total, dtype=torch.float64, device=dev) if want_mass else None
+        esp = None
+        if _MODES[mode] == _lib.MODE_TABLE:
+            if npart_esp is None:
+                raise ValueError("table mode needs NPART_ESP")
+            esp = np.ascontiguousarray(npart_esp, dtype=np.int32)
+            if esp.shape != (N_ESP,):
+                raise ValueError("NPART_ESP must have 9 entries")
+        ga = _GeomArgs(geom)
+        o = self._opts(mode, reduction, path, u1_ld, u11_ld)
+        st = _lib.load().asohf_hip_density(
+            self._h, ga.ref, n, _ptr(x), _ptr(y), _ptr(z), _ptr(m),
+            _np_ptr(esp, C.c_int32) if esp is not None else None, _ptr(u1), _ptr(u11), _ptr(gm),
+            C.byref(o))
+        _lib.check(self._h, st)
+        return DensityResult(u1, u11, gm)
+
+    def density_host(self, geom: Geometry, x, y, z, m, npart_esp=None, *, mode="table",
+                     reduction="deterministic", u1=None, u11=None, u1_ld=None, u11_ld=None):
+        """The host-array entry point (the one the Fortran shim calls): numpy in, numpy out."""
+        s1, n11 = self.out_shapes(geom, u1_ld, u11_ld)
+        u1 = np.zeros(s1, np.float32) if u1 is None else u1
+        u11 = np.zeros(n11, np.float32) if u11 is None else u11
+        gm = np.zeros(1 + geom.npatch_total, np.float64)
+        x, y, z, m = (np.ascontiguousarray(a, np.float32) for a in (x, y, z, m))
+        esp = np.ascontiguousarray(npart_esp if npart_esp is not None else np.zeros(9), np.int32)
+        ga = _GeomArgs(geom)
+        o = self._opts(mode, reduction, "batched", u1_ld, u11_ld)
+        o.stream = None
+        f = C.c_float
+        st = _lib.load().asohf_hip_density_host(
+            self._h, ga.ref, len(x), x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p),
+            z.ctypes.data_as(C.c_void_p), m.ctypes.data_as(C.c_void_p), _np_ptr(esp, C.c_int32),
+            u1.ctypes.data_as(C.c_void_p), u11.ctypes.data_as(C.c_void_p),
+            gm.ctypes.data_as(C.c_void_p), C.byref(o))
+        _lib.check(self._h, st)
+        del f
+        return u1, u11, gm
+
+    # ------------------------------------------------------------------ adjoint
+    def adjoint(self, geom: Geometry, x, y, z, m, g1, g11, *, u1_ld=None, u11_ld=None):
+        """dL/dm, dL/dx, dL/dy, dL/dz of L = sum(g1*u1) + sum(g11*u11), TSC mode."""
+        n = _check_particles(x, y, z, m)
+        g1 = g1.contiguous().float()
+        g11 = (g11.contiguous().float() if g11 is not None and g11.numel()
+               else torch.zeros(1, dtype=torch.float32, device=x.device))
+        outs = [torch.empty(n, dtype=torch.float32, device=x.device) for _ in range(4)]
+        ga = _GeomArgs(geom)
+        o = self._opts("tsc", "deterministic", "batched", u1_ld, u11_ld)
+        st = _lib.load().asohf_hip_density_adjoint(
+            self._h, ga.ref, n, _ptr(x), _ptr(y), _ptr(z), _ptr(m), _ptr(g1), _ptr(g11),
+            *[_ptr(t) for t in outs], C.byref(o))
+        _lib.check(self._h, st)
+        return tuple(outs)
+
+    # ------------------------------------------------------------------ helpers
+    def membership(self, geom: Geometry, x, y, z, npart_esp=None, *, mode="tsc"):
+        """(grid, particle, ix, jy, kz) of every box member, as the binning
+        stage computes them (int32 [count, 5], unordered)."""
+        n = _check_particles(x, y, z)
+        esp = np.ascontiguousarray(npart_esp if npart_esp is not None else np.zeros(9), np.int32)
+        ga = _GeomArgs(geom)
+        L = _lib.load()
+        cnt = C.c_int64()
+        st = L.asohf_hip_membership(self._h, ga.ref, n, _ptr(x), _ptr(y), _ptr(z),
+                                    _np_ptr(esp, C.c_int32), _MODES[mode], None, 0, C.byref(cnt))
+        _lib.check(self._h, st)
+        out = torch.empty((max(cnt.value, 1), 5), dtype=torch.int32, device=x.device)
+        st = L.asohf_hip_membership(self._h, ga.ref, n, _ptr(x), _ptr(y), _ptr(z),
+                                    _np_ptr(esp, C.c_int32), _MODES[mode], _ptr(out), cnt.value,
+                                    C.byref(cnt))
+        _lib.check(self._h, st)
+        return out[: cnt.value]
+
+    def get_tune(self) -> dict:
+        t = _lib.Tune()
+        _lib.check(self._h, _lib.load().asohf_hip_get_tune(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tune(self, tile=(16, 8, 8), block=256, chunk=4096, waves_per_cu=16):
+        t = _lib.Tune()
+        t.tile[0], t.tile[1], t.tile[2] = tile
+        t.block, t.chunk, t.waves_per_cu = block, chunk, waves_per_cu
+        _lib.check(self._h, _lib.load().asohf_hip_set_tune(self._h, C.byref(t)))
+
+    def autotune(self, geom: Geometry, x, y, z, m, npart_esp=None, *, mode="tsc",
+                 reduction="deterministic") -> dict:
+        n = _check_particles(x, y, z, m)
+        s1, n11 = self.out_shapes(geom)
+        u1 = torch.empty(s1, dtype=torch.float32, device=x.device)
+        u11 = torch.empty(n11, dtype=torch.float32, device=x.device)
+        esp = np.ascontiguousarray(npart_esp if npart_esp is not None else np.zeros(9), np.int32)
+        ga = _GeomArgs(geom)
+        o = self._opts(mode, reduction)
+        t = _lib.Tune()
+        st = _lib.load().asohf_hip_autotune(self._h, ga.ref, n, _ptr(x), _ptr(y), _ptr(z), _ptr(m),
+                                            _np_ptr(esp, C.c_int32), _ptr(u1), _ptr(u11), C.byref(o),
+                                            C.byref(t))
+        _lib.check(self._h, st)
+        return t.as_dict()
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        _lib.check(self._h, _lib.load().asohf_hip_get_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+
+_builders: dict[int, DensityBuilder] = {}
+
+
+def builder(device: int | None = None) -> DensityBuilder:
+    d = torch.cuda.current_device() if device is None else int(device)
+    if d not in _builders:
+        _builders[d] = DensityBuilder(d)
+    return _builders[d]
+
+
+def density(geom: Geometry, x, y, z, m, npart_esp=None, **kw) -> DensityResult:
+    """Module-level DENSITY on the tensors' device."""
+    return builder(x.device.index).density(geom, x, y, z, m, npart_esp, **kw)
+
+
+class TSCDeposit(torch.autograd.Function):
+    """TSC deposit (INTERPOLATE_DENSITY_TSC semantics) with the hand-written
+    adjoint: d(loss)/d(positions, masses).  Weights are recomputed from the
+    positions in the backward pass, not stored."""
+
+    @staticmethod
+    def forward(ctx, x, y, z, m, geom, bld, reduction):
+        res = bld.density(geom, x, y, z, m, None, mode="tsc", reduction=reduction, want_mass=False)
+        ctx.save_for_backward(x, y, z, m)
+        ctx.geom, ctx.bld = geom, bld
+        return res.u1, res.u11
+
+    @staticmethod
+    def backward(ctx, g1, g11):
+        x, y, z, m = ctx.saved_tensors
+        if g1 is None:
+            g1 = torch.zeros((ctx.geom.nz, ctx.geom.ny, ctx.geom.nx), dtype=torch.float32, device=x.device)
+        dm, dx, dy, dz = ctx.bld.adjoint(ctx.geom, x, y, z, m, g1, g11)
+        return dx, dy, dz, dm, None, None, None
+
+
+def tsc_deposit(x, y, z, m, geom: Geometry, reduction: str = "deterministic", bld=None):
+    """Differentiable TSC density: (u1 [nz,ny,nx], u11 packed)."""
+    bld = bld or builder(x.device.index)
+    return TSCDeposit.apply(x, y, z, m, geom, bld, reduction)

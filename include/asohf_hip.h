@@ -1,0 +1,174 @@
+/*
+ * asohf_hip.h -- C ABI of libasohf_hip, the MI355X (gfx950) density build of
+ * ASOHF: particle-to-mesh assignment onto the base grid and every AMR patch.
+ *
+ * All entry points are extern "C", take plain pointers and sizes, return an
+ * asohf_status, and never abort.  A context is single-threaded: one host
+ * thread at a time, one HIP stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   SUBROUTINE DENSITY(...)              grids.f:710-994  (+ COMPUTE_KERNELS
+ *                                        grids.f:997-1067, RENORM_DENSITY
+ *                                        grids.f:2668-2722), called once per
+ *                                        snapshot at asohf.f:467-470
+ *   SUBROUTINE INTERPOLATE_DENSITY_TSC   grids.f:1070-1398
+ */
+#ifndef ASOHF_HIP_H
+#define ASOHF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct asohf_ctx asohf_ctx;
+
+typedef enum {
+  ASOHF_OK = 0,
+  ASOHF_ERR_INVALID = -1,   /* bad argument, geometry or particle outside the wrap range */
+  ASOHF_ERR_CAPACITY = -2,  /* a compile-time-style bound of the reference exceeded      */
+  ASOHF_ERR_HIP = -3,       /* a HIP runtime call failed                                 */
+  ASOHF_ERR_OOM = -4,       /* device allocation failed                                  */
+  ASOHF_ERR_NODEVICE = -5   /* no HIP device                                             */
+} asohf_status;
+
+/* Deposit semantics. */
+enum {
+  ASOHF_MODE_TABLE = 1, /* DENSITY: NGP cell + normalised kernel table of width
+                           2*2^max(IR-species,0)+1 (grids.f:710-994)           */
+  ASOHF_MODE_TSC = 2    /* INTERPOLATE_DENSITY_TSC: position-dependent 27-cell
+                           TSC on every level (grids.f:1070-1398)              */
+};
+/* Reduction of the cell sums. */
+enum {
+  ASOHF_REDUCE_DETERMINISTIC = 0, /* exact int64 fixed-point sums: bit-identical
+                                     on every run, mass-conserving on the base grid */
+  ASOHF_REDUCE_ATOMIC = 1         /* FP32 LDS atomics: run-to-run order varies      */
+};
+/* Execution form. */
+enum {
+  ASOHF_PATH_BATCHED = 0, /* all grids of all levels in one persistent launch  */
+  ASOHF_PATH_PLAIN = 1    /* one launch per grid, global FP32 atomics           */
+};
+
+/* Geometry: the DENSITY arguments and COMMON blocks (grids.f:710-742).  Host
+ * pointers.  Patches are numbered level-major as in the reference: level IR
+ * holds SUM(NPATCH(0:IR-1)) .. SUM(NPATCH(0:IR))-1 (0-based). */
+typedef struct asohf_geom {
+  int32_t nx, ny, nz;      /* NX, NY, NZ                                   */
+  int32_t nl;              /* NL (0..9)                                    */
+  const int32_t *npatch;   /* NPATCH(0:NL), nl+1 entries                   */
+  const int32_t *patchnx, *patchny, *patchnz; /* PATCHNX/NY/NZ [P]         */
+  const float *patchrx, *patchry, *patchrz;   /* PATCHRX/RY/RZ [P]         */
+  float dx, dy, dz;        /* COMMON /ESPACIADO/                           */
+  float lado0;             /* LADO0                                        */
+  float rote, rete;        /* COMMON /BACK/                                */
+  float fdm;               /* RENORM_DENSITY divisor (1 = none)            */
+  int32_t interp_degree;   /* COMPUTE_KERNELS: 2 TSC-like, 1 CIC-like      */
+  const float *radx, *rady, *radz; /* COMMON /GRID/ RADX(0:NX+1) etc., or
+                                      NULL: rebuilt as MALLA does with
+                                      LADO = LADO0 - LADO0/NX             */
+} asohf_geom;
+
+typedef struct asohf_opts {
+  int32_t mode;        /* ASOHF_MODE_TABLE or ASOHF_MODE_TSC                 */
+  int32_t reduction;   /* ASOHF_REDUCE_*                                     */
+  int32_t path;        /* ASOHF_PATH_*                                       */
+  int32_t use_graph;   /* 1: replay the batched pipeline as a hipGraph       */
+  int32_t u1_ld[2];    /* leading dims of u1 (NMAX, NMAY); 0 = (nx, ny)       */
+  int32_t u11_ld[3];   /* NAMRX, NAMRY, NAMRZ of a padded U11; 0 = packed:
+                          patch p at offset sum_{q<p} n1*n2*n3, x fastest    */
+  void *stream;        /* hipStream_t to run on; NULL = the context's stream */
+} asohf_opts;
+
+/* Tuning of the batched deposit (chosen by asohf_hip_autotune). */
+typedef struct asohf_tune {
+  int32_t tile[3];          /* cells per tile along x, y, z (powers of two)   */
+  int32_t block;            /* threads per workgroup (64..1024)               */
+  int32_t chunk;            /* max entries per work item ("particles per block") */
+  int32_t waves_per_cu;     /* persistent grid = waves_per_cu * CUs workgroups */
+} asohf_tune;
+
+/* Counters and timings of the last density call. */
+typedef struct asohf_stats {
+  int64_t n_particles;
+  int64_t n_pairs;          /* (particle, grid) members under the mode's box rule */
+  int64_t n_entries;        /* binned (particle, tile) entries                     */
+  int64_t n_tiles;
+  int64_t n_items;          /* work items of the persistent deposit                */
+  int64_t n_cells;          /* output cells written                                */
+  double ms_total;          /* device time of the whole call (hipEvents)           */
+  double ms_bin;            /* count + scan + scatter                              */
+  double ms_deposit;        /* persistent deposit + grid sums                      */
+  float level_min[10], level_max[10]; /* per-level min/max of the fields (grids.f:835,987) */
+} asohf_stats;
+
+/* Context. device < 0 selects the current device. */
+asohf_status asohf_hip_create(asohf_ctx **ctx, int32_t device);
+void asohf_hip_destroy(asohf_ctx *ctx);
+const char *asohf_hip_last_error(const asohf_ctx *ctx);
+
+/* The density build on device arrays.
+ *   rx, ry, rz, m : device FP32 [n], particles in reference order (species
+ *                   contiguous, ascending), replacing MODULE PARTICLES'
+ *                   RXPA, RYPA, RZPA, MASAP (particles.f:9-11)
+ *   npart_esp     : host int32 [9], NPART_ESP(0:8); mode TABLE deposits
+ *                   SUM(NPART_ESP) particles, mode TSC all n
+ *   u1            : device FP32 base field, U1 (interior written only)
+ *   u11           : device FP32 patch fields, U11 (interiors written only)
+ *   grid_mass     : device FP64 [1+P] or NULL: deposited mass per grid
+ *                   (base first), summed in fixed order
+ * Replaces DENSITY (grids.f:710) + RENORM_DENSITY (grids.f:2668) in mode
+ * TABLE, INTERPOLATE_DENSITY_TSC (grids.f:1070) + RENORM_DENSITY in TSC. */
+asohf_status asohf_hip_density(asohf_ctx *ctx, const asohf_geom *geom, int64_t n,
+                               const float *rx, const float *ry, const float *rz,
+                               const float *m, const int32_t *npart_esp, float *u1,
+                               float *u11, double *grid_mass, const asohf_opts *opts);
+
+/* Same on host arrays (copies in and out): the form the Fortran DENSITY
+ * shim calls with MODULE PARTICLES and COMMON /VARIA/ U1, U11. */
+asohf_status asohf_hip_density_host(asohf_ctx *ctx, const asohf_geom *geom, int64_t n,
+                                    const float *rx, const float *ry, const float *rz,
+                                    const float *m, const int32_t *npart_esp, float *u1,
+                                    float *u11, double *grid_mass, const asohf_opts *opts);
+
+/* Adjoint of mode TSC (no reference counterpart: the reference has no
+ * derivatives).  Given dL/du1 and dL/du11 (device FP32, same layouts as the
+ * forward outputs), writes dL/dm, dL/dx, dL/dy, dL/dz (device FP32 [n]).
+ * Weights are recomputed from the positions. */
+asohf_status asohf_hip_density_adjoint(asohf_ctx *ctx, const asohf_geom *geom, int64_t n,
+                                       const float *rx, const float *ry, const float *rz,
+                                       const float *m, const float *g1, const float *g11,
+                                       float *dm, float *dxp, float *dyp, float *dzp,
+                                       const asohf_opts *opts);
+
+/* Membership records of the binning stage, for bit-exact index checks:
+ * out (device int32 [cap][5]) = {grid, particle, ix, jy, kz} with grid 0 the
+ * base grid, 1+p patch p, and the reference's own cell index expressions
+ * (mode TABLE: grids.f:798-800, 912-914; mode TSC: grids.f:1180-1192,
+ * 1321-1323).  Order is unspecified; *count receives the total. */
+asohf_status asohf_hip_membership(asohf_ctx *ctx, const asohf_geom *geom, int64_t n,
+                                  const float *rx, const float *ry, const float *rz,
+                                  const int32_t *npart_esp, int32_t mode, int32_t *out,
+                                  int64_t cap, int64_t *count);
+
+/* Tuning. */
+asohf_status asohf_hip_get_tune(const asohf_ctx *ctx, asohf_tune *out);
+asohf_status asohf_hip_set_tune(asohf_ctx *ctx, const asohf_tune *tune);
+/* Times the batched pipeline over a fixed candidate set on these inputs and
+ * keeps the fastest; *out receives it. */
+asohf_status asohf_hip_autotune(asohf_ctx *ctx, const asohf_geom *geom, int64_t n,
+                                const float *rx, const float *ry, const float *rz,
+                                const float *m, const int32_t *npart_esp, float *u1,
+                                float *u11, const asohf_opts *opts, asohf_tune *out);
+
+asohf_status asohf_hip_get_stats(const asohf_ctx *ctx, asohf_stats *out);
+
+/* Version string of the library. */
+const char *asohf_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

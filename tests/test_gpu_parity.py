@@ -1,0 +1,191 @@
+"""Parity of the HIP density build against the oracle (GPU).
+
+Bars (the tolerance is written in each test):
+  * cell / patch indices of the binning stage: bit-exact with the reference's
+    FP32 expressions (oracle.pairs, itself bit-exact with the reference);
+  * deterministic mode: max relative error <= 1e-5 against the FP64 oracle on
+    cells above the field's mean; base-grid mass conserved to 1e-12; the
+    fields bit-identical between runs and between tile shapes;
+  * atomic mode: within the FP32 accumulation error of the reference itself
+    (twice the reference restatement's own error against FP64, at least 1e-5).
+"""
+import numpy as np
+import pytest
+import torch
+
+from asohf_amd import synth
+from oracle import oracle as O
+
+from conftest import FIXTURES, load_fixture
+
+pytestmark = pytest.mark.gpu
+
+MODES = {"table": "A", "tsc": "B"}
+
+
+@pytest.fixture(scope="module")
+def bld():
+    from asohf_amd.density import DensityBuilder
+    b = DensityBuilder(0)
+    yield b
+    b.close()
+
+
+def dev(snap):
+    d = torch.device("cuda:0")
+    return tuple(torch.from_numpy(np.ascontiguousarray(a)).to(d) for a in (snap.x, snap.y, snap.z, snap.m))
+
+
+def rel_err_above_mean(got, ref):
+    got = np.asarray(got, np.float64).ravel()
+    ref = np.asarray(ref, np.float64).ravel()
+    if ref.size == 0:
+        return 0.0, 0.0
+    sel = ref > ref.mean()
+    if not sel.any():
+        return 0.0, 0.0
+    r = np.abs(got[sel] - ref[sel]) / ref[sel]
+    return float(r.max()), float(np.sqrt((r ** 2).mean()))
+
+
+def run(bld, snap, mode, red="deterministic", path="batched", **kw):
+    x, y, z, m = dev(snap)
+    r = bld.density(snap.geom, x, y, z, m, snap.npart_esp, mode=mode, reduction=red, path=path, **kw)
+    torch.cuda.synchronize()
+    return r.u1.cpu().numpy(), r.u11.cpu().numpy(), r.grid_mass.cpu().numpy()
+
+
+def atomic_tol(snap, mode):
+    u1, u11 = O.density32(snap, MODES[mode])
+    d1, d11, _ = O.density64(snap, MODES[mode])
+    e = max(rel_err_above_mean(u1, d1)[0], rel_err_above_mean(u11, d11)[0])
+    return max(1e-5, 2.0 * e)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("mode", ["table", "tsc"])
+@pytest.mark.parametrize("red", ["deterministic", "atomic"])
+def test_fields_vs_fp64_oracle(bld, name, mode, red):
+    snap, _ = load_fixture(name)
+    print(f"case {name} {mode} {red}", flush=True)
+    u1, u11, gm = run(bld, snap, mode, red)
+    d1, d11, dgm = O.density64(snap, MODES[mode])
+    tol = 1e-5 if red == "deterministic" else atomic_tol(snap, mode)
+    e1, _ = rel_err_above_mean(u1, d1)
+    e11, _ = rel_err_above_mean(u11, d11[: u11.size] if d11.size else d11)
+    assert e1 <= tol and e11 <= tol, (e1, e11, tol)
+    total = float(snap.m[: snap.npart_esp.sum()].astype(np.float64).sum()) if mode == "table" \
+        else float(snap.m.astype(np.float64).sum())
+    if red == "deterministic":
+        assert abs(gm[0] - total) <= 1e-12 * total
+    else:
+        assert abs(gm[0] - total) <= 1e-5 * total
+    # per-patch deposited mass against the FP64 oracle
+    if len(gm) > 1:
+        np.testing.assert_allclose(gm[1:], dgm[1:], rtol=1e-5, atol=1e-6 * total)
+
+
+@pytest.mark.parametrize("mode", ["table", "tsc"])
+def test_membership_indices_bitexact(bld, golden, mode):
+    snap, _ = golden
+    x, y, z, _ = dev(snap)
+    got = bld.membership(snap.geom, x, y, z, snap.npart_esp, mode=mode).cpu().numpy()
+    n_use = int(snap.npart_esp.sum()) if mode == "table" else snap.n
+    want = O.pairs(snap, MODES[mode], all_members=True)
+    want = want[want[:, 1] < n_use]
+    key = lambda a: np.lexsort((a[:, 4], a[:, 3], a[:, 2], a[:, 1], a[:, 0]))
+    got, want = got[key(got)], want[key(want)]
+    assert got.shape == want.shape
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("mode", ["table", "tsc"])
+def test_deterministic_bit_identical_across_runs_and_tiles(bld, mode):
+    snap, _ = load_fixture("seeded_13_cic")
+    ref = run(bld, snap, mode)
+    ref2 = run(bld, snap, mode)
+    assert all(np.array_equal(a, b) for a, b in zip(ref, ref2))
+    old = bld.get_tune()
+    try:
+        for tile in ((8, 8, 8), (16, 16, 16)):
+            for chunk in (64, 4096):
+                bld.set_tune(tile=tile, chunk=chunk, waves_per_cu=8)
+                got = run(bld, snap, mode)
+                assert all(np.array_equal(a, b) for a, b in zip(ref, got)), (tile, chunk)
+    finally:
+        bld.set_tune(**old)
+
+
+@pytest.mark.parametrize("mode", ["table", "tsc"])
+def test_plain_form_vs_oracle(bld, golden, mode):
+    snap, _ = golden
+    u1, u11, gm = run(bld, snap, mode, "atomic", "plain")
+    d1, d11, _ = O.density64(snap, MODES[mode])
+    tol = atomic_tol(snap, mode)
+    assert rel_err_above_mean(u1, d1)[0] <= tol
+    assert rel_err_above_mean(u11, d11)[0] <= tol
+
+
+def test_padded_layout_matches_packed(bld):
+    snap, _ = load_fixture("seeded_11")
+    g = snap.geom
+    u1, u11, _ = run(bld, snap, "table")
+    namr = (int(g.pnx.max()) + 2, int(g.pny.max()) + 1, int(g.pnz.max()) + 3)
+    ld = (g.nx + 3, g.ny + 1)
+    x, y, z, m = dev(snap)
+    s1, n11 = bld.out_shapes(g, ld, namr)
+    pad1 = torch.full(s1, -1.0, device="cuda:0")
+    pad11 = torch.full((n11,), -1.0, device="cuda:0")
+    bld.density(g, x, y, z, m, snap.npart_esp, mode="table", u1=pad1, u11=pad11, u1_ld=ld, u11_ld=namr)
+    p1, p11 = pad1.cpu().numpy(), pad11.cpu().numpy()
+    assert np.array_equal(p1[:, :g.ny, :g.nx], u1)
+    assert np.all(p1[:, g.ny:, :] == -1) and np.all(p1[:, :, g.nx:] == -1)
+    off = g.patch_offsets()
+    blk = p11.reshape(g.npatch_total, namr[2], namr[1], namr[0])
+    for p in range(g.npatch_total):
+        a, b, c = g.pnx[p], g.pny[p], g.pnz[p]
+        want = u11[off[p]:off[p + 1]].reshape(c, b, a)
+        assert np.array_equal(blk[p, :c, :b, :a], want)
+        untouched = blk[p].copy()
+        untouched[:c, :b, :a] = -1
+        assert np.all(untouched == -1)
+
+
+def test_host_entry_point_matches_device(bld):
+    snap, _ = load_fixture("edges")
+    u1, u11, gm = run(bld, snap, "table")
+    h1, h11, hgm = bld.density_host(snap.geom, snap.x, snap.y, snap.z, snap.m, snap.npart_esp, mode="table")
+    assert np.array_equal(h1, u1) and np.array_equal(h11, u11) and np.array_equal(hgm, gm)
+
+
+def test_edge_cases_empty_and_no_levels(bld):
+    from asohf_amd._lib import AsohfError
+    snap, _ = load_fixture("seeded_12")
+    empty = synth.Snapshot(snap.x[:0], snap.y[:0], snap.z[:0], snap.m[:0], np.zeros(9, np.int32), snap.geom)
+    for mode in ("table", "tsc"):
+        u1, u11, gm = run(bld, empty, mode)
+        assert not u1.any() and not u11.any() and not gm.any()
+    # NaN position and out-of-wrap position are rejected, not written
+    bad = synth.Snapshot(snap.x.copy(), snap.y, snap.z, snap.m, snap.npart_esp, snap.geom)
+    bad.x[5] = np.nan
+    with pytest.raises(AsohfError):
+        run(bld, bad, "tsc")
+    far = synth.Snapshot(snap.x.copy(), snap.y, snap.z, snap.m, snap.npart_esp, snap.geom)
+    far.x[3] = np.float32(10 * snap.geom.lado0)
+    with pytest.raises(AsohfError):
+        run(bld, far, "table")
+    run(bld, far, "tsc")   # the TSC base index is clamped (grids.f:1184-1192): accepted
+
+
+def test_larger_snapshot_properties(bld):
+    snap = synth.make_snapshot(synth.CONFIGS[1], n_side=64, nx=64, patches_per_level=(30, 25, 20), seed=4)
+    for mode in ("table", "tsc"):
+        print(f"case larger {mode}", flush=True)
+        u1, u11, gm = run(bld, snap, mode)
+        d1, d11, dgm = O.density64(snap, MODES[mode], nthreads=0)
+        assert rel_err_above_mean(u1, d1)[0] <= 1e-5
+        assert rel_err_above_mean(u11, d11)[0] <= 1e-5
+        tot = float(snap.m.astype(np.float64).sum())
+        assert abs(gm[0] - tot) <= 1e-12 * tot
+        a1, a11, _ = run(bld, snap, mode, "atomic")
+        assert rel_err_above_mean(a1, d1)[0] <= atomic_tol(snap, mode)
