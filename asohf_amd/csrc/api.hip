@@ -96,9 +96,17 @@ void compute_kernels(int nl, int degree, std::vector<float>& K) {
   }
 }
 
+// content hash of the geometry arrays (8 bytes per step)
 uint64_t fnv(uint64_t h, const void* p, size_t n) {
   const unsigned char* c = (const unsigned char*)p;
-  for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    memcpy(&w, c + i, 8);
+    h = (h ^ w) * 0x100000001b3ull;
+    h ^= h >> 29;
+  }
+  for (; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
   return h;
 }
 
@@ -137,9 +145,10 @@ struct asohf_ctx {
   DBuf h_in[4], h_out[2], h_mass;    // device copies for the host entry point
   Counters* hcount = nullptr;          // pinned
   unsigned long long* hscalar = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[8] = {};   // 0 start, 1 count pass, 2 scatter start, 3 scatter end, 4/5 deposit, 6 end
   asohf_stats stats{};
   bool stats_pending = false;
+  bool stats_phases = false;
   std::map<std::pair<int, int>, std::vector<float>> ktab_cache;
 };
 
@@ -526,6 +535,7 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   HIPCHK(launch_plan_items(ctx->start.as<unsigned long long>(), ntiles, chunk, chunk_wide, 0, nullptr,
                            ctx->nitems.as<int>(), ctx->split.as<int>(), C, s),
          "planning items");
+  HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
   HIPCHK(hipMemcpyAsync(ctx->hcount, C, sizeof(Counters), hipMemcpyDeviceToHost, s), "reading counters");
   HIPCHK(hipMemcpyAsync(ctx->hscalar, ctx->start.as<unsigned long long>() + nkeys, sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, s), "reading totals");
@@ -545,10 +555,11 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   HIPCHK(ctx->items.ensure(sizeof(Item) * std::max<unsigned>(hc.n_items, 1)), "allocating work items");
   HIPCHK(ctx->scratch.ensure((size_t)std::max<unsigned>(hc.n_split, 1) * T * (red == 0 ? 8 : 4)),
          "allocating split-tile scratch");
+  HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
   HIPCHK(launch_scatter(P, A, ctx->cursor.as<unsigned long long>(), ctx->entries.as<int4>(), n_entries,
                         ctx->cus, s),
          "scatter kernel");
-  HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
+  HIPCHK(hipEventRecord(ctx->ev[3], s), "event");
   HIPCHK(launch_plan_items(ctx->start.as<unsigned long long>(), ntiles, chunk, chunk_wide, 1,
                            ctx->items.as<Item>(), ctx->nitems.as<int>(), ctx->split.as<int>(), C, s),
          "writing items");
@@ -573,9 +584,11 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   const int block = ctx->tune.block;
   int grid = std::max(1, ctx->cus * waves / std::max(block / 64, 1));
   grid = (int)std::min<long long>(grid, std::max<unsigned>(hc.n_items, 1));
+  HIPCHK(hipEventRecord(ctx->ev[4], s), "event");
   HIPCHK(launch_deposit(P, D, red, grid, block, s), "deposit kernel");
+  HIPCHK(hipEventRecord(ctx->ev[5], s), "event");
   if (grid_mass) HIPCHK(launch_grid_sums(P, D, red, grid_mass, s), "grid sums");
-  HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
+  HIPCHK(hipEventRecord(ctx->ev[6], s), "event");
   ctx->stats = asohf_stats{};
   ctx->stats.n_particles = A.n;
   ctx->stats.n_pairs = (int64_t)hc.n_pairs;
@@ -584,6 +597,7 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   ctx->stats.n_items = hc.n_items;
   ctx->stats.n_cells = H.ncells;
   ctx->stats_pending = true;
+  ctx->stats_phases = true;
   return ASOHF_OK;
 }
 
@@ -631,12 +645,13 @@ static asohf_status run_plain(asohf_ctx* ctx, const asohf_opts& o, const Parts& 
     }
   }
   HIPCHK(launch_plain_finish(P, u1, u11, grid_mass, s), "normalisation");
-  HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
+  HIPCHK(hipEventRecord(ctx->ev[6], s), "event");
   ctx->stats = asohf_stats{};
   ctx->stats.n_particles = A.n;
   ctx->stats.n_tiles = 0;
   ctx->stats.n_cells = ctx->plan.ncells;
   ctx->stats_pending = true;
+  ctx->stats_phases = false;
   return ASOHF_OK;
 }
 
@@ -715,7 +730,7 @@ asohf_status asohf_hip_density(asohf_ctx* ctx, const asohf_geom* geom, int64_t n
   if (st != ASOHF_OK) return st;
   st = build_plan(ctx, geom, o.mode, &o);
   if (st != ASOHF_OK) return st;
-  hipStream_t s = o.stream ? (hipStream_t)o.stream : ctx->stream;
+  hipStream_t s = (hipStream_t)o.stream;   // NULL: the default stream
   if (o.path == ASOHF_PATH_PLAIN) return run_plain(ctx, o, A, geom, u1, u11, grid_mass, s);
   return run_batched(ctx, o, A, u1, u11, grid_mass, s);
 }
@@ -735,7 +750,7 @@ asohf_status asohf_hip_density_host(asohf_ctx* ctx, const asohf_geom* geom, int6
   st = build_plan(ctx, geom, o.mode, &o);
   if (st != ASOHF_OK) return st;
   const HostPlan& H = ctx->plan;
-  hipStream_t s = o.stream ? (hipStream_t)o.stream : ctx->stream;
+  hipStream_t s = o.stream ? (hipStream_t)o.stream : ctx->stream;   // host arrays: the context's stream
   const size_t pb = sizeof(float) * std::max<int64_t>(n, 1);
   const float* src[4] = {rx, ry, rz, m};
   for (int k = 0; k < 4; ++k) {
@@ -784,14 +799,14 @@ asohf_status asohf_hip_density_adjoint(asohf_ctx* ctx, const asohf_geom* geom, i
   Parts A;
   st = make_parts(ctx, o, n, rx, ry, rz, m, nullptr, A);
   if (st != ASOHF_OK) return st;
-  hipStream_t s = o.stream ? (hipStream_t)o.stream : ctx->stream;
+  hipStream_t s = (hipStream_t)o.stream;   // NULL: the default stream
   HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
   if (n > 0) HIPCHK(launch_adjoint(ctx->plan.dev, A, g1, g11, dm, dxp, dyp, dzp, ctx->cus, s), "adjoint kernel");
-  HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
-  HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
+  HIPCHK(hipEventRecord(ctx->ev[6], s), "event");
   ctx->stats = asohf_stats{};
   ctx->stats.n_particles = n;
   ctx->stats_pending = true;
+  ctx->stats_phases = false;
   return ASOHF_OK;
 }
 
@@ -880,13 +895,24 @@ asohf_status asohf_hip_get_stats(const asohf_ctx* cctx, asohf_stats* out) {
   if (!cctx || !out) return ASOHF_ERR_INVALID;
   asohf_ctx* ctx = const_cast<asohf_ctx*>(cctx);
   if (ctx->stats_pending) {
-    HIPCHK(hipEventSynchronize(ctx->ev[2]), "waiting for the call");
-    float a = 0, b = 0;
-    HIPCHK(hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[2]), "timing");
-    HIPCHK(hipEventElapsedTime(&b, ctx->ev[0], ctx->ev[1]), "timing");
-    ctx->stats.ms_total = a;
-    ctx->stats.ms_bin = b;
-    ctx->stats.ms_deposit = a - b;
+    HIPCHK(hipEventSynchronize(ctx->ev[6]), "waiting for the call");
+    auto ms = [&](int a, int b, double* out) -> hipError_t {
+      float v = 0;
+      hipError_t e = hipEventElapsedTime(&v, ctx->ev[a], ctx->ev[b]);
+      *out = v;
+      return e;
+    };
+    HIPCHK(ms(0, 6, &ctx->stats.ms_total), "timing");
+    if (ctx->stats_phases) {
+      double t4 = 0;
+      HIPCHK(ms(0, 1, &ctx->stats.ms_bin), "timing");
+      HIPCHK(ms(2, 3, &ctx->stats.ms_scatter), "timing");
+      HIPCHK(ms(4, 5, &ctx->stats.ms_deposit), "timing");
+      HIPCHK(ms(3, 4, &t4), "timing");
+      double t6 = 0;
+      HIPCHK(ms(5, 6, &t6), "timing");
+      ctx->stats.ms_sums = t4 + t6;
+    }
     ctx->stats_pending = false;
   }
   *out = ctx->stats;

@@ -17,13 +17,16 @@ NMAX-sized grid; here the grid is NX-sized.)
 
 AMR hierarchy: patches that obey CREATE_MESH's geometric rules
 (grids.f:186-313, 457-624): even sides, a level-1 patch starts on a base cell
-I1 with PATCHRX = RADX(I1) (grids.f:300-302); a level-IR patch lies inside its
+I1 with PATCHRX = RADX(I1) (grids.f:300-302); a level-IR patch lies inside one
 parent and starts on the parent's cell I1 with
-PATCHRX = PATCHRX(parent) + (I1-1.5)*DXPA(parent) (grids.f:586-588); same-level
-overlaps are allowed.  Placement follows the particle density: every patch is
-centred near a halo drawn with probability ~ n_h, at a radius that shrinks with
-the level.  Sides are log-uniform in [8, 64] fine cells per axis, clipped to
-the parent.
+PATCHRX = PATCHRX(parent) + (I1-1.5)*DXPA(parent) (grids.f:586-588).
+Placement follows CREATE_MESH's refinement criterion (REFINE_THR = 8 particles
+per parent cell, grids.f:107-129, 357-405, i.e. density above 8**IR times the
+mean at level IR): each halo's refinable region is the cube out to its NFW
+radius of that density, tiled inside each parent by non-overlapping patches
+with sides log-uniform in [8, 64] fine cells.  Overlaps only arise where the
+regions of neighbouring haloes meet.  The threshold is scaled by one factor
+per workload so the total patch count lands near the target.
 """
 from __future__ import annotations
 
@@ -167,80 +170,155 @@ def species_local_density(x, y, z, m, geom: Geometry, rodo, re0) -> np.ndarray:
     return s
 
 
-def hierarchy(spec: SnapshotSpec, geom_base: dict, halos, rng: np.random.Generator):
-    """Patch arrays obeying CREATE_MESH's nesting and lattice rules."""
+def _nfw_overdensity_radius(rhos_over_thr: np.ndarray) -> np.ndarray:
+    """x = r/r_s where the NFW density rho_s/(x(1+x)^2) falls to the threshold:
+    solves x(1+x)^2 = rho_s/threshold by bisection (0 where never reached)."""
+    lo = np.zeros_like(rhos_over_thr)
+    hi = np.maximum(np.cbrt(rhos_over_thr), rhos_over_thr) + 1.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        f = mid * (1.0 + mid) ** 2
+        above = f < rhos_over_thr
+        lo = np.where(above, mid, lo)
+        hi = np.where(above, hi, mid)
+    return 0.5 * (lo + hi)
+
+
+def _segments(total: int, rng, lo_fine: int, hi_fine: int) -> list:
+    """Split `total` parent cells into runs whose fine-cell sides (2 per parent
+    cell) are log-uniform in [lo_fine, hi_fine]; a short remainder is merged
+    into the previous run (or the two are split evenly) so no run is below
+    lo_fine/2 parent cells when total allows it."""
+    lo_r, hi_r = max(lo_fine // 2, 1), max(hi_fine // 2, 1)
+    out, left = [], total
+    while left > 0:
+        side = int(np.exp(rng.uniform(np.log(lo_fine), np.log(hi_fine))))
+        run = min(max(side // 2, lo_r), left)
+        out.append(run)
+        left -= run
+    if len(out) > 1 and out[-1] < lo_r:
+        rest = out.pop()
+        if out[-1] + rest <= hi_r:
+            out[-1] += rest
+        else:
+            both = out.pop() + rest
+            out += [both // 2, both - both // 2]
+    return out
+
+
+def hierarchy(spec: SnapshotSpec, geom_base: dict, halos, rng: np.random.Generator, n: int,
+              alpha: float = 1.0):
+    """Patch arrays built the way CREATE_MESH refines: a level-IR patch covers
+    parent cells that hold at least REFINE_THR = 8 particles, i.e. where the
+    density exceeds 8**IR times the mean (alpha scales that threshold).  For
+    every halo the refinable region is the cube around its centre out to the
+    NFW radius of that density; inside each parent patch (the base grid at
+    level 1) the region is tiled by non-overlapping patches with sides
+    log-uniform in [side_lo, side_hi] fine cells."""
     nh, conc, centre, rvir = halos
     nx = geom_base["nx"]
     radx, dx = geom_base["radx"], geom_base["dx"]
-    nl = spec.nl
-    p_halo = nh / max(nh.sum(), 1)
-    # per level: arrays of (pnx, pny, pnz, i1, j1, k1, prx, pry, prz, parent, halo)
+    L = spec.box
+    rs = rvir / conc
+    mu_c = np.log1p(conc) - conc / (1.0 + conc)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rhos = np.where(nh > 0, nh * L ** 3 / (n * 4.0 * np.pi * rs ** 3 * mu_c), 0.0)
+    left_edge = float(radx[1]) - float(dx) / 2.0         # -L/2 up to FP32 rounding
+    # per level: list of dicts (n[3], i1[3] in parent cells, r[3] FP32, parent idx, halo)
     levels = []
-    logs = np.log(spec.side_lo), np.log(spec.side_hi)
-    for ir in range(1, nl + 1):
-        want = spec.patches_per_level[ir - 1]
+    for ir in range(1, spec.nl + 1):
+        thr = alpha * 8.0 ** ir
+        rad = np.minimum(_nfw_overdensity_radius(rhos / thr) * rs, rvir)
+        dxp = f32(dx / f32(2.0) ** (ir - 1))          # parent cell size
+        cur = []
         if ir == 1:
-            par_n = None
-            halo = rng.choice(len(nh), size=want, p=p_halo)
-            parent = np.zeros(want, dtype=np.int64)
+            parents = [(-1, h) for h in np.nonzero(2 * rad / (dx / 2.0) >= 4)[0]]
         else:
-            prev = levels[-1]
-            if len(prev["halo"]) == 0:
-                levels.append({k: v[:0] for k, v in prev.items()})
+            parents = [(k, p["halo"]) for k, p in enumerate(levels[-1])]
+        for pk, h in parents:
+            c, r = centre[h], rad[h]
+            if 2 * r / (float(dxp) / 2.0) < 4:
                 continue
-            w = nh[prev["halo"]].astype(np.float64)
-            w /= w.sum()
-            pick = rng.choice(len(prev["halo"]), size=want, p=w)
-            halo = prev["halo"][pick]
-            parent = pick
-        dxp = f32(dx / f32(2.0) ** (ir - 1))      # parent cell size
-        sides = np.exp(rng.uniform(logs[0], logs[1], size=(want, 3)))
-        sides = np.clip((np.round(sides / 2) * 2).astype(np.int64), spec.side_lo, spec.side_hi)
-        # target point: halo centre + offset shrinking with the level
-        sig = rvir[halo] * 0.6 / (1.5 ** (ir - 1))
-        pt = centre[halo] + rng.normal(size=(want, 3)) * sig[:, None]
-        out = {k: np.zeros(want, dtype=np.int64) for k in ("nx", "ny", "nz", "i1", "j1", "k1")}
-        rr = np.zeros((want, 3), dtype=np.float32)
-        for ax, (nk, ik) in enumerate((("nx", "i1"), ("ny", "j1"), ("nz", "k1"))):
-            if ir == 1:
-                ext = nx                                   # parent cells available
-                lo_edge = f32(radx[1] - dx / f32(2.0))
-                cell = np.floor((pt[:, ax] - lo_edge) / dx).astype(np.int64) + 1
-                border = 1
-            else:
-                prev = levels[-1]
-                ext = prev["n" + "xyz"[ax]][parent]
-                porg = prev["r"][parent, ax] - dxp        # parent left edge
-                cell = np.floor((pt[:, ax] - porg) / dxp).astype(np.int64) + 1
-                border = 0
-            s = np.minimum(sides[:, ax], 2 * (ext - 2 * border))
-            s = np.maximum(s - (s % 2), 2)
-            npc = s // 2                                   # parent cells covered
-            i1 = cell - npc // 2
-            i1 = np.clip(i1, 1 + border, ext - border - npc + 1)
-            out[nk] = s
-            out[ik] = i1
-            if ir == 1:
-                rr[:, ax] = radx[i1]                       # grids.f:300-302
-            else:
-                prr = prev["r"][parent, ax].astype(np.float32)
-                rr[:, ax] = prr + (i1.astype(np.float32) - f32(1.5)) * dxp  # grids.f:586-588
-        levels.append({"nx": out["nx"], "ny": out["ny"], "nz": out["nz"],
-                       "r": rr, "parent": parent, "halo": halo})
-    npatch = [0] + [len(l["halo"]) for l in levels]
-    base = np.cumsum([0] + npatch)              # global 0-based start of level ir at base[ir]
-    pnx = np.concatenate([l["nx"] for l in levels]) if levels else np.zeros(0, np.int64)
-    pny = np.concatenate([l["ny"] for l in levels]) if levels else np.zeros(0, np.int64)
-    pnz = np.concatenate([l["nz"] for l in levels]) if levels else np.zeros(0, np.int64)
-    r = np.concatenate([l["r"] for l in levels]) if levels else np.zeros((0, 3), np.float32)
+            box = []
+            for a in range(3):
+                if pk < 0:
+                    org, ext = left_edge, nx                  # base cells 1..nx
+                    lo_cell = max(int(np.floor((c[a] - r - org) / float(dxp))) + 1, 2)
+                    hi_cell = min(int(np.floor((c[a] + r - org) / float(dxp))) + 1, ext - 1)
+                else:
+                    par = levels[-1][pk]
+                    org = float(par["r"][a]) - float(dxp)     # parent's left edge (its XLFIX)
+                    ext = par["n"][a]
+                    lo_cell = max(int(np.floor((c[a] - r - org) / float(dxp))) + 1, 1)
+                    hi_cell = min(int(np.floor((c[a] + r - org) / float(dxp))) + 1, ext)
+                if hi_cell < lo_cell:
+                    box = None
+                    break
+                # CREATE_MESH's minimum patch size: at least side_lo fine cells
+                minrun = spec.side_lo // 2
+                first = 2 if pk < 0 else 1
+                if hi_cell - lo_cell + 1 < minrun and ext - first + 1 >= minrun:
+                    need = minrun - (hi_cell - lo_cell + 1)
+                    lo_cell -= need // 2
+                    hi_cell += need - need // 2
+                    if lo_cell < first:
+                        hi_cell += first - lo_cell
+                        lo_cell = first
+                    last = ext - 1 if pk < 0 else ext
+                    if hi_cell > last:
+                        lo_cell -= hi_cell - last
+                        hi_cell = last
+                box.append((lo_cell, hi_cell))
+            if box is None:
+                continue
+            segs = [_segments(b1 - b0 + 1, rng, spec.side_lo, spec.side_hi) for b0, b1 in box]
+            starts = [np.concatenate([[b0], b0 + np.cumsum(sg)[:-1]]) for (b0, _), sg in zip(box, segs)]
+            for kz, sz in enumerate(segs[2]):
+                for ky, sy in enumerate(segs[1]):
+                    for kx, sx in enumerate(segs[0]):
+                        i1 = (int(starts[0][kx]), int(starts[1][ky]), int(starts[2][kz]))
+                        nn = (2 * sx, 2 * sy, 2 * sz)
+                        rr = np.zeros(3, np.float32)
+                        for a in range(3):
+                            if pk < 0:
+                                rr[a] = radx[i1[a]]                                     # grids.f:300-302
+                            else:
+                                rr[a] = f32(levels[-1][pk]["r"][a] +
+                                            (f32(i1[a]) - f32(1.5)) * dxp)               # grids.f:586-588
+                        cur.append({"n": nn, "r": rr, "parent": pk, "halo": h})
+        levels.append(cur)
+    npatch = [0] + [len(l) for l in levels]
+    base = np.cumsum([0] + npatch)
+    flat = [p for l in levels for p in l]
+    pn = np.array([p["n"] for p in flat], np.int64).reshape(-1, 3)
+    pr = np.array([p["r"] for p in flat], np.float32).reshape(-1, 3)
     pare = []
     for ir, l in enumerate(levels, start=1):
-        if ir == 1:
-            pare.append(np.zeros(len(l["halo"]), np.int64))
+        for p in l:
+            pare.append(0 if ir == 1 else p["parent"] + base[ir - 1] + 1)   # 1-based global parent
+    return (np.array(npatch, np.int32), pn[:, 0], pn[:, 1], pn[:, 2], pr[:, 0], pr[:, 1], pr[:, 2],
+            np.array(pare, np.int64))
+
+
+def _tuned_hierarchy(spec, geom_base, halos, seed, n):
+    """Scales the refinement threshold so the total patch count lands near the
+    workload's target (sum of patches_per_level)."""
+    target = sum(spec.patches_per_level)
+    lo, hi = -6.0, 6.0          # log2 of the threshold factor
+    best = None
+    for _ in range(12):
+        mid = 0.5 * (lo + hi)
+        h = hierarchy(spec, geom_base, halos, np.random.default_rng(seed + 1000), n, alpha=2.0 ** mid)
+        cnt = int(h[0].sum())
+        if best is None or abs(cnt - target) < abs(int(best[0].sum()) - target):
+            best = h
+        if abs(cnt - target) <= 0.05 * target:
+            break
+        if cnt > target:
+            lo = mid
         else:
-            pare.append(l["parent"] + base[ir - 1] + 1)   # 1-based global parent index
-    pare = np.concatenate(pare) if pare else np.zeros(0, np.int64)
-    return np.array(npatch, np.int32), pnx, pny, pnz, r[:, 0], r[:, 1], r[:, 2], pare
+            hi = mid
+    return best
 
 
 def make_snapshot(spec: SnapshotSpec, n_side: int | None = None, nx: int | None = None,
@@ -262,8 +340,8 @@ def make_snapshot(spec: SnapshotSpec, n_side: int | None = None, nx: int | None 
     lado0 = f32(spec.box)
     g0 = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, np.zeros(1, np.int32),
                   np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0))
-    npatch, pnx, pny, pnz, prx, pry, prz, pare = hierarchy(
-        spec, {"nx": spec.nx, "radx": g0.radx, "dx": g0.dx}, halos, rng)
+    npatch, pnx, pny, pnz, prx, pry, prz, pare = _tuned_hierarchy(
+        spec, {"nx": spec.nx, "radx": g0.radx, "dx": g0.dx}, halos, spec.seed, n)
     geom = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, npatch, pnx, pny, pnz,
                     prx, pry, prz, pare=pare, interp_degree=interp_degree)
     x = pos[:, 0].astype(np.float32)

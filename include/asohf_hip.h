@@ -79,7 +79,9 @@ typedef struct asohf_opts {
   int32_t u1_ld[2];    /* leading dims of u1 (NMAX, NMAY); 0 = (nx, ny)       */
   int32_t u11_ld[3];   /* NAMRX, NAMRY, NAMRZ of a padded U11; 0 = packed:
                           patch p at offset sum_{q<p} n1*n2*n3, x fastest    */
-  void *stream;        /* hipStream_t to run on; NULL = the context's stream */
+  void *stream;        /* hipStream_t to run on; NULL = the default (null)
+                          stream.  asohf_hip_density_host uses the
+                          context's own stream when NULL.                    */
 } asohf_opts;
 
 /* Tuning of the batched deposit (chosen by asohf_hip_autotune). */
@@ -99,8 +101,10 @@ typedef struct asohf_stats {
   int64_t n_items;          /* work items of the persistent deposit                */
   int64_t n_cells;          /* output cells written                                */
   double ms_total;          /* device time of the whole call (hipEvents)           */
-  double ms_bin;            /* count + scan + scatter                              */
-  double ms_deposit;        /* persistent deposit + grid sums                      */
+  double ms_bin;            /* count + scan + item planning (before the host read) */
+  double ms_scatter;        /* scatter of the entries                              */
+  double ms_deposit;        /* the persistent deposit kernel alone                 */
+  double ms_sums;           /* item writing, scratch clearing and grid sums        */
   float level_min[10], level_max[10]; /* per-level min/max of the fields (grids.f:835,987) */
 } asohf_stats;
 

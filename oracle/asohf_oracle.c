@@ -576,15 +576,36 @@ int orc_density_b32(const orc_geom *g, int64_t n, const float *x, const float *y
   float denbas = g->dx * g->dy * g->dz * g->rote * powi_f(g->rete, 3);
   for (size_t c = 0; c < ncell; ++c) u1[c] = (u1[c] / denbas) / g->fdm;
 
-  /* patches: membership in the BUF=1 box is the whole filter (PLEV >= IR is
-   * implied by it, grids.f:1114-1157 vs 1249-1261, 1318-1320) */
+  /* PLEV (grids.f:1114-1157): the finest level whose BUF=1 box holds the
+   * particle; level IR then scans only particles with PLEV >= IR, in their
+   * original order (grids.f:1249-1261).  Membership in the patch's own box
+   * implies PLEV >= IR, so this is a pure filter of the per-patch scan. */
   int64_t *off = patch_offsets(g);
+  int *plev = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+  for (int ir = nl; ir >= 1; --ir) {
+    float p2 = powi_f(2.0f, ir);
+    float dxpa = g->dx / p2, dypa = g->dy / p2, dzpa = g->dz / p2;
+    int lo, hi;
+    level_range(g, ir, &lo, &hi);
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1)
+    for (int64_t i = 0; i < n; ++i) {
+      if (plev[i] != 0) continue;
+      for (int p = lo; p < hi; ++p) {
+        b_box b = b_patch_box(g, p, dxpa, dypa, dzpa);
+        if (b_in_box(&b, x[i], y[i], z[i])) { plev[i] = ir; break; }
+      }
+    }
+  }
+  int64_t *lst = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
   for (int ir = 1; ir <= nl; ++ir) {
     float p2 = powi_f(2.0f, ir);
     float dxpa = g->dx / p2, dypa = g->dy / p2, dzpa = g->dz / p2;
     float den = dxpa * dypa * dzpa * g->rote * powi_f(g->rete, 3);
     int lo, hi;
     level_range(g, ir, &lo, &hi);
+    int64_t nbas = 0;
+    for (int64_t i = 0; i < n; ++i)
+      if (plev[i] >= ir) lst[nbas++] = i;
 #pragma omp parallel for schedule(dynamic) num_threads(nt) if (nt > 1)
     for (int p = lo; p < hi; ++p) {
       const int n1 = g->pnx[p], n2 = g->pny[p], n3 = g->pnz[p];
@@ -592,7 +613,8 @@ int orc_density_b32(const orc_geom *g, int64_t n, const float *x, const float *y
       size_t nc = (size_t)n1 * n2 * n3;
       memset(u, 0, sizeof(float) * nc);
       b_box b = b_patch_box(g, p, dxpa, dypa, dzpa);
-      for (int64_t i = 0; i < n; ++i) {
+      for (int64_t q = 0; q < nbas; ++q) {
+        const int64_t i = lst[q];
         float xp = x[i], yp = y[i], zp = z[i];
         if (!b_in_box(&b, xp, yp, zp)) continue;
         int ix = (int)((xp - b.xl) / dxpa) + 1 - 1;
@@ -619,6 +641,8 @@ int orc_density_b32(const orc_geom *g, int64_t n, const float *x, const float *y
       for (size_t c = 0; c < nc; ++c) u[c] = (u[c] / den) / g->fdm;
     }
   }
+  free(lst);
+  free(plev);
   free(off);
   free(rs.own);
   return 0;
