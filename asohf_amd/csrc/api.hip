@@ -270,6 +270,7 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
       B.org[a] = -g->lado0 / 2.0f;
     }
     B.d[0] = g->dx; B.d[1] = g->dy; B.d[2] = g->dz;
+    for (int a = 0; a < 3; ++a) B.inv_d[a] = 1.0f / B.d[a];
     B.den = g->dx * g->dy * g->dz * g->rote * r3;         // DENBAS (grids.f:824)
     B.tile0 = 0;
     ntiles = B.nt[0] * B.nt[1] * B.nt[2];
@@ -290,6 +291,7 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
       G.n[0] = g->patchnx[p]; G.n[1] = g->patchny[p]; G.n[2] = g->patchnz[p];
       G.level = l;
       G.d[0] = dxpa; G.d[1] = dypa; G.d[2] = dzpa;
+      for (int a = 0; a < 3; ++a) G.inv_d[a] = 1.0f / G.d[a];
       G.den = den;
       const float pr[3] = {g->patchrx[p], g->patchry[p], g->patchrz[p]};
       const float d[3] = {dxpa, dypa, dzpa};

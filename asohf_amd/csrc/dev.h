@@ -31,6 +31,14 @@ __device__ __forceinline__ void tsc_weights(float xp, float c0, float c1, float 
   b = fabsf(xp - c1) / d; w[1] = 0.75f - b * b;
   b = fabsf(xp - c2) / d; t = 1.5f - b; w[2] = 0.5f * (t * t);
 }
+// the same weights with the division by the cell size replaced by a multiply
+// by its FP32 reciprocal (each b within 1.5 ulp of the reference's)
+__device__ __forceinline__ void tsc_weights_r(float xp, float c0, float c1, float c2, float inv_d, float w[3]) {
+  float b, t;
+  b = fabsf(xp - c0) * inv_d; t = 1.5f - b; w[0] = 0.5f * (t * t);
+  b = fabsf(xp - c1) * inv_d; w[1] = 0.75f - b * b;
+  b = fabsf(xp - c2) * inv_d; t = 1.5f - b; w[2] = 0.5f * (t * t);
+}
 // derivatives of the three weights with respect to xp
 __device__ __forceinline__ void tsc_dweights(float xp, float c0, float c1, float c2, float d, float dw[3]) {
   float t;

@@ -25,6 +25,7 @@ struct GridDesc {
   float org[3];      // table mode: XLFIX (patch) / XL (base); TSC mode: XL of the BUF box / XL (base)
   float pr[3];       // PATCHRX/RY/RZ (TSC cell centres MINIRADX)
   float d[3];        // cell size DXPA/DYPA/DZPA (DX/DY/DZ on the base grid)
+  float inv_d[3];    // 1/d in FP32 (weights only; indices use the IEEE division)
   float den;         // DENBAS of the level, FP32 as the reference forms it
   int ld1;           // output row stride (elements)
   int ld2;           // output plane stride (elements)

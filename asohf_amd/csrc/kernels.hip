@@ -67,13 +67,56 @@ __global__ void fill_u32_kernel(unsigned* p, unsigned v, long long n) {
 // ---------------------------------------------------------------------------
 // count / scatter
 // ---------------------------------------------------------------------------
+// Block-local aggregation of (tile) keys.  A workgroup takes a contiguous
+// run of particles (consecutive particles of one halo share tiles) and counts
+// its emissions per key in an LDS open-addressing table; only one global
+// atomic per distinct key leaves the workgroup.  A key whose probe path is
+// full is never inserted later (the table only fills), so every emission of
+// that key takes the direct global path consistently in all phases.
+constexpr int kBinThreads = 256;
+constexpr int kBinPer = 4;                       // particles per thread per block
+constexpr int kBinChunk = kBinThreads * kBinPer;
+constexpr int kBinCap = 2048;                    // LDS table slots (power of two)
+constexpr int kBinProbe = 32;
+
+__device__ __forceinline__ unsigned key_hash(unsigned k) {
+  k ^= k >> 16; k *= 0x7feb352du; k ^= k >> 15; k *= 0x846ca68bu; k ^= k >> 16;
+  return k & (kBinCap - 1);
+}
+// slot of key k, inserting it when absent; -1 when its probe path is full
+__device__ __forceinline__ int lds_insert(int* keys, int k) {
+  unsigned h = key_hash((unsigned)k);
+  for (int p = 0; p < kBinProbe; ++p) {
+    const int old = atomicCAS(&keys[h], -1, k);
+    if (old == -1 || old == k) return (int)h;
+    h = (h + 1) & (kBinCap - 1);
+  }
+  return -1;
+}
+__device__ __forceinline__ int lds_find(const int* keys, int k) {
+  unsigned h = key_hash((unsigned)k);
+  for (int p = 0; p < kBinProbe; ++p) {
+    const int v = keys[h];
+    if (v == k) return (int)h;
+    if (v == -1) return -1;
+    h = (h + 1) & (kBinCap - 1);
+  }
+  return -1;
+}
+
 template <int MODE>
-__global__ void __launch_bounds__(256) count_kernel(Plan P, Parts A, unsigned* cnt, Counters* C) {
+__global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, unsigned* cnt, Counters* C) {
+  __shared__ int keys[kBinCap];
+  __shared__ unsigned vals[kBinCap];
+  for (int k = threadIdx.x; k < kBinCap; k += kBinThreads) { keys[k] = -1; vals[k] = 0u; }
+  __syncthreads();
   unsigned err = 0;
   unsigned long long pairs = 0;
   float mmax = 0.f;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
-       i += (long long)gridDim.x * blockDim.x) {
+  const long long c0 = (long long)blockIdx.x * kBinChunk;
+  for (int r = 0; r < kBinPer; ++r) {
+    const long long i = c0 + (long long)r * kBinThreads + threadIdx.x;
+    if (i >= A.n) break;
     const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
     if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) { err |= kErrNonFinite; continue; }
     if (m < 0.f) { err |= kErrNegMass; continue; }
@@ -81,9 +124,17 @@ __global__ void __launch_bounds__(256) count_kernel(Plan P, Parts A, unsigned* c
     const int s = MODE == 1 ? species_of(A, i) : 0;
     enumerate_particle<MODE>(
         P, i, x, y, z, m, s,
-        [&](int, int t, int wide, int4) { atomicAdd(&cnt[2 * t + wide], 1u); },
+        [&](int, int t, int wide, int4) {
+          const int key = 2 * t + wide;
+          const int h = lds_insert(keys, key);
+          if (h >= 0) atomicAdd(&vals[h], 1u);
+          else atomicAdd(&cnt[key], 1u);
+        },
         [&](int, int, int, int) { ++pairs; }, err);
   }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kBinCap; k += kBinThreads)
+    if (keys[k] >= 0) atomicAdd(&cnt[keys[k]], vals[k]);
   pairs = wave_sum(pairs);
   mmax = wave_max(mmax);
   err = wave_or(err);
@@ -95,18 +146,51 @@ __global__ void __launch_bounds__(256) count_kernel(Plan P, Parts A, unsigned* c
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(256) scatter_kernel(Plan P, Parts A, unsigned long long* cursor,
-                                                      int4* entries, unsigned long long cap) {
+__global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, unsigned long long* cursor,
+                                                              int4* entries, unsigned long long cap) {
+  __shared__ int keys[kBinCap];
+  __shared__ unsigned vals[kBinCap];
+  __shared__ unsigned long long base[kBinCap];
+  for (int k = threadIdx.x; k < kBinCap; k += kBinThreads) { keys[k] = -1; vals[k] = 0u; }
+  __syncthreads();
   unsigned err = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
-       i += (long long)gridDim.x * blockDim.x) {
+  const long long c0 = (long long)blockIdx.x * kBinChunk;
+  // phase A: count this block's emissions per key
+  for (int r = 0; r < kBinPer; ++r) {
+    const long long i = c0 + (long long)r * kBinThreads + threadIdx.x;
+    if (i >= A.n) break;
+    const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
+    if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m)) || m < 0.f) continue;
+    const int s = MODE == 1 ? species_of(A, i) : 0;
+    enumerate_particle<MODE>(
+        P, i, x, y, z, m, s,
+        [&](int, int t, int wide, int4) {
+          const int h = lds_insert(keys, 2 * t + wide);
+          if (h >= 0) atomicAdd(&vals[h], 1u);
+        },
+        [&](int, int, int, int) {}, err);
+  }
+  __syncthreads();
+  // phase B: one global reservation per key
+  for (int k = threadIdx.x; k < kBinCap; k += kBinThreads) {
+    if (keys[k] >= 0) base[k] = atomicAdd(&cursor[keys[k]], (unsigned long long)vals[k]);
+    vals[k] = 0u;
+  }
+  __syncthreads();
+  // phase C: the same emissions again, written to their reserved slots
+  for (int r = 0; r < kBinPer; ++r) {
+    const long long i = c0 + (long long)r * kBinThreads + threadIdx.x;
+    if (i >= A.n) break;
     const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
     if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m)) || m < 0.f) continue;
     const int s = MODE == 1 ? species_of(A, i) : 0;
     enumerate_particle<MODE>(
         P, i, x, y, z, m, s,
         [&](int, int t, int wide, int4 e) {
-          const unsigned long long slot = atomicAdd(&cursor[2 * t + wide], 1ull);
+          const int key = 2 * t + wide;
+          const int h = lds_find(keys, key);
+          const unsigned long long slot =
+              h >= 0 ? base[h] + atomicAdd(&vals[h], 1u) : atomicAdd(&cursor[key], 1ull);
           if (slot < cap) entries[slot] = e;   // the count pass sized the buffer exactly
         },
         [&](int, int, int, int) {}, err);
@@ -374,14 +458,14 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.n[a]);
-              tsc_weights(p[a], rad[a][ci[a] - 1], rad[a][ci[a]], rad[a][ci[a] + 1], G.d[a], w[a]);
+              tsc_weights_r(p[a], rad[a][ci[a] - 1], rad[a][ci[a]], rad[a][ci[a] + 1], G.inv_d[a], w[a]);
             }
           } else {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               ci[a] = (int)((p[a] - G.org[a]) / G.d[a]);
-              tsc_weights(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
-                          minirad(G.pr[a], ci[a] + 1, G.d[a]), G.d[a], w[a]);
+              tsc_weights_r(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
+                            minirad(G.pr[a], ci[a] + 1, G.d[a]), G.inv_d[a], w[a]);
             }
           }
         } else {
@@ -803,7 +887,9 @@ hipError_t launch_fill_u32(unsigned* p, unsigned v, long long n, hipStream_t s) 
 }
 
 hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* C, int cus, hipStream_t s) {
-  const int grid = grid_for(A.n, 256, cus * 16);
+  (void)cus;
+  if (A.n <= 0) return hipSuccess;
+  const int grid = (int)((A.n + kBinChunk - 1) / kBinChunk);
   if (P.mode == 2) count_kernel<2><<<grid, 256, 0, s>>>(P, A, cnt, C);
   else count_kernel<1><<<grid, 256, 0, s>>>(P, A, cnt, C);
   return hipGetLastError();
@@ -811,7 +897,9 @@ hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* 
 
 hipError_t launch_scatter(const Plan& P, const Parts& A, unsigned long long* cursor, int4* entries,
                           unsigned long long cap, int cus, hipStream_t s) {
-  const int grid = grid_for(A.n, 256, cus * 16);
+  (void)cus;
+  if (A.n <= 0) return hipSuccess;
+  const int grid = (int)((A.n + kBinChunk - 1) / kBinChunk);
   if (P.mode == 2) scatter_kernel<2><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap);
   else scatter_kernel<1><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap);
   return hipGetLastError();
