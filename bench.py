@@ -127,6 +127,23 @@ def main() -> int:
     }
     u1_gpu = r.u1.cpu().numpy()
 
+    if args.mode == "tsc":
+        # the hand-written adjoint: d(sum(w*rho))/d(positions, masses)
+        gen = torch.Generator(device=dev).manual_seed(spec.seed)
+        w1 = torch.randn(s1, generator=gen, device=dev)
+        w11 = torch.randn(n11, generator=gen, device=dev)
+        bld.adjoint(g, x, y, z, m, w1, w11)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            bld.adjoint(g, x, y, z, m, w1, w11)
+        torch.cuda.synchronize()
+        aw = 1000.0 * (time.perf_counter() - t0) / args.steps
+        b_adj = 16.0 * n_pairs + 4.0 * n_cells + 16.0 * snap.n
+        out["adjoint"] = {"ms_per_step": aw, "ratio_to_forward": aw / ms_step,
+                          "algorithmic_bytes": b_adj, "achieved_gbs": b_adj / (aw * 1e-3) / 1e9}
+        log(f"[bench] adjoint: {aw:.3f} ms/step ({aw / ms_step:.2f}x forward)")
+
     if not args.no_plain:
         kwp = dict(kw, path="plain", reduction="atomic")
         bld.density(g, x, y, z, m, snap.npart_esp, **kwp)

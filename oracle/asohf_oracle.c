@@ -959,3 +959,75 @@ void orc_stencil_work_a(const orc_geom *g, int64_t n, const float *x, const floa
     out[ir] = w;
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* Mode B loss of ONE particle, FP64: sum_c g_c rho_c over the cells its */
+/* own deposit touches (base grid and every patch whose box holds it).   */
+/* By linearity of the deposit, L(x_i + h) - L(x_i - h) for the full     */
+/* snapshot equals this quantity's difference, which makes central       */
+/* differences cheap at any grid size.  Same formulas as orc_density_b64.*/
+/* ------------------------------------------------------------------ */
+double orc_loss_one_b64(const orc_geom *g, float xp, float yp, float zp, float m, const double *g1,
+                        const double *g11) {
+  const int nx = g->nx, ny = g->ny, nz = g->nz, nl = g->nl;
+  rad_set rs = rad_get(g);
+  const float xl = -g->lado0 / 2.0f, yl = -g->lado0 / 2.0f, zl = -g->lado0 / 2.0f;
+  double L = 0.0;
+  {
+    int ix = b_base_index(xp, xl, g->dx, nx), jy = b_base_index(yp, yl, g->dy, ny),
+        kz = b_base_index(zp, zl, g->dz, nz);
+    double vx[3], vy[3], vz[3];
+    b_weights64(xp, rs.radx + ix - 1, g->dx, vx);
+    b_weights64(yp, rs.rady + jy - 1, g->dy, vy);
+    b_weights64(zp, rs.radz + kz - 1, g->dz, vz);
+    double den0 = (double)g->dx * g->dy * g->dz * g->rote * (double)g->rete * g->rete * g->rete;
+    for (int kk = -1; kk <= 1; ++kk)
+      for (int jj = -1; jj <= 1; ++jj)
+        for (int ii = -1; ii <= 1; ++ii) {
+          int i1 = ix + ii, j1 = jy + jj, k1 = kz + kk;
+          if (i1 == nx + 1) i1 = 1;
+          if (i1 == 0) i1 = nx;
+          if (j1 == ny + 1) j1 = 1;
+          if (j1 == 0) j1 = ny;
+          if (k1 == nz + 1) k1 = 1;
+          if (k1 == 0) k1 = nz;
+          L += g1[cidx(i1, j1, k1, nx, ny)] * ((double)m * vx[ii + 1] * vy[jj + 1] * vz[kk + 1]) / den0 /
+               (double)g->fdm;
+        }
+  }
+  int64_t *off = patch_offsets(g);
+  for (int ir = 1; ir <= nl; ++ir) {
+    float p2 = powi_f(2.0f, ir);
+    float dxpa = g->dx / p2, dypa = g->dy / p2, dzpa = g->dz / p2;
+    double den = (double)dxpa * dypa * dzpa * g->rote * (double)g->rete * g->rete * g->rete;
+    int lo, hi;
+    level_range(g, ir, &lo, &hi);
+    for (int p = lo; p < hi; ++p) {
+      b_box b = b_patch_box(g, p, dxpa, dypa, dzpa);
+      if (!b_in_box(&b, xp, yp, zp)) continue;
+      const int n1 = g->pnx[p], n2 = g->pny[p], n3 = g->pnz[p];
+      int ix = (int)((xp - b.xl) / dxpa), jy = (int)((yp - b.yl) / dypa), kz = (int)((zp - b.zl) / dzpa);
+      float rx[3], ry[3], rz[3];
+      for (int t = 0; t < 3; ++t) {
+        rx[t] = b_minirad(g->prx[p], ix - 1 + t, dxpa);
+        ry[t] = b_minirad(g->pry[p], jy - 1 + t, dypa);
+        rz[t] = b_minirad(g->prz[p], kz - 1 + t, dzpa);
+      }
+      double vx[3], vy[3], vz[3];
+      b_weights64(xp, rx, dxpa, vx);
+      b_weights64(yp, ry, dypa, vy);
+      b_weights64(zp, rz, dzpa, vz);
+      for (int kk = -1; kk <= 1; ++kk)
+        for (int jj = -1; jj <= 1; ++jj)
+          for (int ii = -1; ii <= 1; ++ii) {
+            int i1 = ix + ii, j1 = jy + jj, k1 = kz + kk;
+            if (i1 > 0 && i1 <= n1 && j1 > 0 && j1 <= n2 && k1 > 0 && k1 <= n3)
+              L += g11[off[p] + cidx(i1, j1, k1, n1, n2)] * ((double)m * vx[ii + 1] * vy[jj + 1] * vz[kk + 1]) /
+                   den / (double)g->fdm;
+          }
+    }
+  }
+  free(off);
+  free(rs.own);
+  return L;
+}

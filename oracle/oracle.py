@@ -67,6 +67,8 @@ def lib():
                                       _dp, _dp, _dp, _dp, C.c_int]
         L.orc_pairs.restype = C.c_int64
         L.orc_pairs.argtypes = [G, C.c_int, C.c_int64, _fp, _fp, _fp, _ip, _ip, C.c_int64]
+        L.orc_loss_one_b64.restype = C.c_double
+        L.orc_loss_one_b64.argtypes = [G, C.c_float, C.c_float, C.c_float, C.c_float, _dp, _dp]
         L.orc_stencil_work_a.argtypes = [G, C.c_int64, _fp, _fp, _fp, _ip, C.POINTER(C.c_int64)]
         for nm in ("orc_density_a32", "orc_density_a64", "orc_density_b32", "orc_density_b64",
                    "orc_adjoint_b64"):
@@ -179,6 +181,17 @@ def adjoint64(snap, g1: np.ndarray, g11: np.ndarray, nthreads: int = 0):
     if rc != 0:
         raise ValueError("oracle rejected the input")
     return tuple(outs)
+
+
+def loss_one(snap, i: int, g1: np.ndarray, g11: np.ndarray, pos=None) -> float:
+    """FP64 loss sum(g*rho) of particle i's own TSC deposit (optionally at a
+    moved position pos = (x, y, z))."""
+    g = _G(snap.geom)
+    x, y, z = pos if pos is not None else (snap.x[i], snap.y[i], snap.z[i])
+    g1 = np.ascontiguousarray(g1, np.float64)
+    g11 = np.ascontiguousarray(g11 if g11.size else np.zeros(1), np.float64)
+    return float(lib().orc_loss_one_b64(g.ref, np.float32(x), np.float32(y), np.float32(z),
+                                         np.float32(snap.m[i]), _p(g1, _dp), _p(g11, _dp)))
 
 
 def pairs(snap, mode: str = "B", all_members: bool = False) -> np.ndarray:
