@@ -454,6 +454,28 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   D.hmask = (unsigned)(cap - 1);
   for (int a = 0; a < 3; ++a) D.tile[a] = T[a];
   D.mode = mode;
+  // are the centre tables MALLA's recurrence?  (always, unless the caller
+  // passed its own COMMON /GRID/ values that differ)
+  D.rad_malla = 1;
+  {
+    const std::vector<float>* rv[3] = {&H.radx, &H.rady, &H.radz};
+    const int nn[3] = {g->nx, g->ny, g->nz};
+    for (int a = 0; a < 3; ++a) {
+      const std::vector<float>& r = *rv[a];
+      const int n = nn[a];
+      const float r1 = r[1];
+      const float c = r[2] - r[1] == 0.f ? 0.f : (r[n] - r[1]) / (float)(n - 1);
+      // recover MALLA's C exactly: C = (B-A)/(N-1) with A = -LADO/2, B = LADO/2
+      const float cm = (lado / 2.0f - (-lado / 2.0f)) / (float)(n - 1);
+      (void)c;
+      D.rad1[a] = r1;
+      D.radc[a] = cm;
+      bool same = true;
+      for (int i = 0; i <= n && same; ++i) same = r[i] == r1 + (float)(i - 1) * cm;
+      same = same && r[n + 1] == (r1 + (float)(n - 1) * cm) + cm;
+      if (!same) D.rad_malla = 0;
+    }
+  }
   D.xl = -g->lado0 / 2.0f;
   D.fdm = g->fdm;
   H.key = key;

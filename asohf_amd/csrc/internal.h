@@ -61,6 +61,9 @@ struct Plan {
   LevelHash lh[kNLevels + 1];
   float xl;                  // -LADO0/2
   float fdm;
+  int rad_malla;             // 1: RAD tables are exactly MALLA's recurrence (rad1 + (i-1)*radc)
+  float rad1[3];
+  float radc[3];
 };
 
 // Particle inputs of one call.

@@ -365,51 +365,64 @@ __device__ __forceinline__ float glb_load(float* a) {
   return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// 27-cell deposit of one particle into the LDS tile.  cx/cy/cz: the three
-// grid cells per axis (already wrapped on the base grid; 0 = outside the
-// patch), w*: the three weights per axis.  Deterministic mode quantises every
-// FP32 term ((m*wx)*wy)*wz to an integer of the fixed-point scale and gives
-// the centre cell the remainder, so each particle deposits exactly Q(m).
+// 27-cell deposit of one particle into the LDS tile.  lx/ly/lz: the tile-local
+// index of the three cells per axis, vx/vy/vz: 3-bit masks of which of them
+// lie in this tile (and in the grid), w*: the weights per axis.  The terms are
+// the reference's FP32 products ((m*wx)*wy)*wz.  Deterministic mode scales the
+// mass by the power-of-two fixed-point scale first (exact), converts every
+// term to an integer and gives the centre cell the remainder Q(m) - (sum of
+// the other 26), so each particle deposits exactly Q(m) on a periodic grid.
 template <int RED, int TX, int TY, int TZ>
 __device__ __forceinline__ void deposit27(typename AccT<RED>::T* acc, float m, const int lx[3],
-                                          const int ly[3], const int lz[3], const float wx[3],
-                                          const float wy[3], const float wz[3], float scale) {
+                                          const int ly[3], const int lz[3], unsigned vx, unsigned vy,
+                                          unsigned vz, const float wx[3], const float wy[3],
+                                          const float wz[3], float scale) {
+  const float ms = RED == 0 ? m * scale : m;
+  float mx[3], mxy[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) mx[a] = ms * wx[a];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) mxy[b][a] = mx[a] * wy[b];
   if (RED == 1) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if ((unsigned)lz[c] >= (unsigned)TZ) continue;
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        if ((unsigned)ly[b] >= (unsigned)TY) continue;
-        const float mxy[3] = {m * wx[0] * wy[b], m * wx[1] * wy[b], m * wx[2] * wy[b]};
+      for (int b = 0; b < 3; ++b)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          if ((unsigned)lx[a] >= (unsigned)TX) continue;
-          lds_add(reinterpret_cast<float*>(acc) + (lx[a] + TX * (ly[b] + TY * lz[c])), mxy[a] * wz[c]);
-        }
-      }
-    }
+        for (int a = 0; a < 3; ++a)
+          if ((vx >> a) & (vy >> b) & (vz >> c) & 1u)
+            lds_add(reinterpret_cast<float*>(acc) + (lx[a] + TX * (ly[b] + TY * lz[c])), mxy[b][a] * wz[c]);
   } else {
-    const int qm = __float2int_rn(m * scale);
     int rest = 0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int b = 0; b < 3; ++b) {
+      for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           if (a == 1 && b == 1 && c == 1) continue;
-          const int q = __float2int_rn(((m * wx[a]) * wy[b]) * wz[c] * scale);
+          const int q = (int)(mxy[b][a] * wz[c]);
           rest += q;
-          if ((unsigned)lx[a] < (unsigned)TX && (unsigned)ly[b] < (unsigned)TY &&
-              (unsigned)lz[c] < (unsigned)TZ)
+          if ((vx >> a) & (vy >> b) & (vz >> c) & 1u)
             lds_add(reinterpret_cast<long long*>(acc) + (lx[a] + TX * (ly[b] + TY * lz[c])), (long long)q);
         }
-      }
-    }
-    if ((unsigned)lx[1] < (unsigned)TX && (unsigned)ly[1] < (unsigned)TY && (unsigned)lz[1] < (unsigned)TZ)
-      lds_add(reinterpret_cast<long long*>(acc) + (lx[1] + TX * (ly[1] + TY * lz[1])), (long long)(qm - rest));
+    if ((vx >> 1) & (vy >> 1) & (vz >> 1) & 1u)
+      lds_add(reinterpret_cast<long long*>(acc) + (lx[1] + TX * (ly[1] + TY * lz[1])), (long long)((int)ms - rest));
   }
+}
+
+// base-grid cell centre RAD(i), i = 0..n+1: MALLA's own FP32 recurrence
+// (RAD(i) = RAD(1)+(i-1)*C, ghost RAD(n+1) = RAD(n)+C) when the plan found the
+// table to be exactly that, a table read otherwise
+__device__ __forceinline__ float base_centre(const Plan& P, int a, int i, int n) {
+  if (P.rad_malla) {
+    if (i <= n) return P.rad1[a] + (float)(i - 1) * P.radc[a];
+    return (P.rad1[a] + (float)(n - 1) * P.radc[a]) + P.radc[a];
+  }
+  const float* r = a == 0 ? P.radx : (a == 1 ? P.rady : P.radz);
+  return r[i];
 }
 
 template <int MODE, int RED, int TX, int TY, int TZ>
@@ -454,11 +467,11 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
           m = __int_as_float(v.w);
           float* w[3] = {wx, wy, wz};
           if (g == 0) {
-            const float* rad[3] = {P.radx, P.rady, P.radz};
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.n[a]);
-              tsc_weights_r(p[a], rad[a][ci[a] - 1], rad[a][ci[a]], rad[a][ci[a] + 1], G.inv_d[a], w[a]);
+              tsc_weights_r(p[a], base_centre(P, a, ci[a] - 1, G.n[a]), base_centre(P, a, ci[a], G.n[a]),
+                            base_centre(P, a, ci[a] + 1, G.n[a]), G.inv_d[a], w[a]);
             }
           } else {
 #pragma unroll
@@ -475,6 +488,7 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
           for (int k = 0; k < 3; ++k) { wx[k] = k0[k]; wy[k] = k0[k]; wz[k] = k0[k]; }
         }
         int lx[3], ly[3], lz[3];
+        unsigned vmask[3] = {0u, 0u, 0u};
         const int org[3] = {i0, j0, k0c};
         int* l[3] = {lx, ly, lz};
         const int ext[3] = {ex, ey, ez};
@@ -486,11 +500,13 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
             bool ok = true;
             if (g == 0) c = wrap1(c, G.n[a]);
             else ok = c >= 1 && c <= G.n[a];
-            int loc = c - org[a];
-            l[a][k] = (ok && loc >= 0 && loc < ext[a]) ? loc : 0x40000000;
+            const int loc = c - org[a];
+            ok = ok && loc >= 0 && loc < ext[a];
+            l[a][k] = loc;
+            vmask[a] |= (ok ? 1u : 0u) << k;
           }
         }
-        deposit27<RED, TX, TY, TZ>(acc, m, lx, ly, lz, wx, wy, wz, scale);
+        deposit27<RED, TX, TY, TZ>(acc, m, lx, ly, lz, vmask[0], vmask[1], vmask[2], wx, wy, wz, scale);
       }
     } else if (MODE == 1) {
       // ---- wide table-mode entries: one wave per entry, lanes over cells ----
@@ -509,9 +525,10 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
         for (int c = lane; c < total; c += 64) {
           const int a = c % bx, r = c / bx, b = r % by, cc = r / by;
           const int ii = xa + a, jj = ya + b, kk = za + cc;
-          const float term = ((m * K[ii - ci[0]]) * K[jj - ci[1]]) * K[kk - ci[2]];
+          const float ms = RED == 0 ? m * scale : m;
+          const float term = ((ms * K[ii - ci[0]]) * K[jj - ci[1]]) * K[kk - ci[2]];
           Acc* dst = acc + ((ii - i0) + TX * ((jj - j0) + TY * (kk - k0c)));
-          if (RED == 0) lds_add(reinterpret_cast<long long*>(dst), (long long)__float2int_rn(term * scale));
+          if (RED == 0) lds_add(reinterpret_cast<long long*>(dst), (long long)(int)term);
           else lds_add(reinterpret_cast<float*>(dst), term);
         }
       }
