@@ -409,6 +409,33 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
     H.hkeys[s] = ukeys[i];
     H.hvals[s] = uvals[i];
   }
+  // nesting: does every level-l membership box lie inside some level-(l-1)
+  // box (FP32 values, as the device compares them)?  Then a particle outside
+  // every level-l box cannot be in a deeper one.
+  D.nested = 1;
+  for (int l = 2; l <= nl && D.nested; ++l) {
+    const LevelHash& lh = D.lh[l - 1];
+    for (int p = level_start[l]; p < level_start[l + 1] && D.nested; ++p) {
+      const GridDesc& Cg = H.grids[1 + p];
+      int b[3];
+      for (int a = 0; a < 3; ++a) {
+        const double mid = 0.5 * ((double)Cg.lo[a] + (double)Cg.hi[a]);
+        b[a] = std::min(std::max((int)std::floor((mid - lh.x0) * (double)lh.inv_bs), 0), lh.nb - 1);
+      }
+      const uint64_t key = bucket_key(l - 1, b[0], b[1], b[2]);
+      bool inside = false;
+      auto itk = std::lower_bound(ukeys.begin(), ukeys.end(), key);
+      if (itk != ukeys.end() && *itk == key) {
+        const int2 se = uvals[itk - ukeys.begin()];
+        for (int q = 0; q < se.y && !inside; ++q) {
+          const GridDesc& Q = H.grids[H.hlist[se.x + q]];
+          inside = Q.lo[0] <= Cg.lo[0] && Cg.hi[0] <= Q.hi[0] && Q.lo[1] <= Cg.lo[1] &&
+                   Cg.hi[1] <= Q.hi[1] && Q.lo[2] <= Cg.lo[2] && Cg.hi[2] <= Q.hi[2];
+        }
+      }
+      if (!inside) D.nested = 0;
+    }
+  }
   if (H.hlist.empty()) H.hlist.push_back(0);
   // ---- upload ----
   size_t off = 0;
@@ -568,13 +595,14 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   const unsigned long long n_entries = *ctx->hscalar;
   asohf_status st = decode_err(ctx, hc.err);
   if (st != ASOHF_OK) return st;
-  // fixed-point scale: a power of two with max(m) * scale <= 2^30, so every
-  // quantised term fits an int32 and any sum of <= 2^33 particles an int64
+  // fixed-point scale: the power of two with max(m) * scale in [2^30, 2^31),
+  // so every quantised term fits an int32 and any sum over <= 2^32 particles
+  // an int64
   const float mmax = __builtin_bit_cast(float, hc.mmax_bits);
   int e2 = 0;
   if (mmax > 0.f) (void)std::frexp(mmax, &e2);
-  const double scale = std::ldexp(1.0, 30 - e2);
-  if (A.n > (1ll << 33)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^33 particles");
+  const double scale = std::ldexp(1.0, 31 - e2);
+  if (A.n > (1ll << 32)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^32 particles");
   HIPCHK(ctx->entries.ensure(sizeof(int4) * std::max<unsigned long long>(n_entries, 1)), "allocating entries");
   HIPCHK(ctx->items.ensure(sizeof(Item) * std::max<unsigned>(hc.n_items, 1)), "allocating work items");
   HIPCHK(ctx->scratch.ensure((size_t)std::max<unsigned>(hc.n_split, 1) * T * (red == 0 ? 8 : 4)),
@@ -587,8 +615,6 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   HIPCHK(launch_plan_items(ctx->start.as<unsigned long long>(), ntiles, chunk, chunk_wide, 1,
                            ctx->items.as<Item>(), ctx->nitems.as<int>(), ctx->split.as<int>(), C, s),
          "writing items");
-  HIPCHK(launch_zero_split(ctx->scratch.p, red, T, (long long)std::max<unsigned>(hc.n_split, 1) * T, C, s),
-         "clearing scratch");
   DepArgs D;
   memset(&D, 0, sizeof(D));
   D.entries = ctx->entries.as<int4>();

@@ -163,13 +163,18 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
           emit(0, B.tile0 + (tz[c] * B.nt[1] + ty[b]) * B.nt[0] + tx[a], 0, e);
   }
   // ---------------- patches, level by level ----------------
+  // When every patch's membership box lies inside a box of the level above
+  // (checked on the host in FP32, P.nested), a particle in no level-l box is
+  // in no deeper box either and the descent stops.
   for (int l = 1; l <= P.nl; ++l) {
     const int2 se = bucket_lookup(P, l, x, y, z);
+    bool any = false;
     for (int q = 0; q < se.y; ++q) {
       const int g = P.hlist[se.x + q];
       const GridDesc& G = P.grids[g];
       if (!(G.lo[0] < x && x < G.hi[0] && G.lo[1] < y && y < G.hi[1] && G.lo[2] < z && z < G.hi[2]))
         continue;
+      any = true;
       int lo[3], hi[3], ci[3];
       int irk = 0;
       if (MODE == 2) {
@@ -202,6 +207,7 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
           for (int a = t0x; a <= t1x; ++a)
             emit(g, G.tile0 + (c * G.nt[1] + b) * G.nt[0] + a, wide, e);
     }
+    if (!any && P.nested) break;
   }
 }
 

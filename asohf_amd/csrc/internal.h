@@ -61,6 +61,7 @@ struct Plan {
   LevelHash lh[kNLevels + 1];
   float xl;                  // -LADO0/2
   float fdm;
+  int nested;                // 1: every level-l box lies inside a level-(l-1) box (FP32 check)
   int rad_malla;             // 1: RAD tables are exactly MALLA's recurrence (rad1 + (i-1)*radc)
   float rad1[3];
   float radc[3];

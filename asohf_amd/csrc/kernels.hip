@@ -104,6 +104,20 @@ __device__ __forceinline__ int lds_find(const int* keys, int k) {
   return -1;
 }
 
+// one particle of lookahead: the next particle's loads are in flight while
+// the current one is enumerated
+struct Prefetch {
+  float x, y, z, m;
+  __device__ Prefetch(const Parts& A, long long i) : x(0.f), y(0.f), z(0.f), m(0.f) {
+    if (i < A.n) { x = A.x[i]; y = A.y[i]; z = A.z[i]; m = A.m[i]; }
+  }
+  __device__ __forceinline__ void next(const Parts& A, long long inext, bool more, float& cx, float& cy,
+                                       float& cz, float& cm) {
+    cx = x; cy = y; cz = z; cm = m;
+    if (more && inext < A.n) { x = A.x[inext]; y = A.y[inext]; z = A.z[inext]; m = A.m[inext]; }
+  }
+};
+
 template <int MODE>
 __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, unsigned* cnt, Counters* C) {
   __shared__ int keys[kBinCap];
@@ -114,10 +128,12 @@ __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, uns
   unsigned long long pairs = 0;
   float mmax = 0.f;
   const long long c0 = (long long)blockIdx.x * kBinChunk;
+  Prefetch pf(A, c0 + threadIdx.x);
   for (int r = 0; r < kBinPer; ++r) {
     const long long i = c0 + (long long)r * kBinThreads + threadIdx.x;
     if (i >= A.n) break;
-    const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
+    float x, y, z, m;
+    pf.next(A, i + kBinThreads, r + 1 < kBinPer, x, y, z, m);
     if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) { err |= kErrNonFinite; continue; }
     if (m < 0.f) { err |= kErrNegMass; continue; }
     mmax = fmaxf(mmax, m);
@@ -156,10 +172,12 @@ __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, u
   unsigned err = 0;
   const long long c0 = (long long)blockIdx.x * kBinChunk;
   // phase A: count this block's emissions per key
+  Prefetch pf(A, c0 + threadIdx.x);
   for (int r = 0; r < kBinPer; ++r) {
     const long long i = c0 + (long long)r * kBinThreads + threadIdx.x;
     if (i >= A.n) break;
-    const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
+    float x, y, z, m;
+    pf.next(A, i + kBinThreads, r + 1 < kBinPer, x, y, z, m);
     if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m)) || m < 0.f) continue;
     const int s = MODE == 1 ? species_of(A, i) : 0;
     enumerate_particle<MODE>(
@@ -178,10 +196,12 @@ __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, u
   }
   __syncthreads();
   // phase C: the same emissions again, written to their reserved slots
+  Prefetch pc(A, c0 + threadIdx.x);
   for (int r = 0; r < kBinPer; ++r) {
     const long long i = c0 + (long long)r * kBinThreads + threadIdx.x;
     if (i >= A.n) break;
-    const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
+    float x, y, z, m;
+    pc.next(A, i + kBinThreads, r + 1 < kBinPer, x, y, z, m);
     if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m)) || m < 0.f) continue;
     const int s = MODE == 1 ? species_of(A, i) : 0;
     enumerate_particle<MODE>(
@@ -319,7 +339,8 @@ __global__ void plan_items(const unsigned long long* start, int ntiles, int chun
   if (!write) {
     atomicAdd(&C->n_items, (unsigned)tot);
     nitems_tile[t] = tot;
-    split_slot[t] = tot > 1 ? (int)atomicAdd(&C->n_split, 1u) : -1;
+    // a split tile owns tot consecutive partial-tile slots
+    split_slot[t] = tot > 1 ? (int)atomicAdd(&C->n_split, (unsigned)tot) : -1;
     return;
   }
   const unsigned base = atomicAdd(&C->n_items_w, (unsigned)tot);
@@ -327,11 +348,11 @@ __global__ void plan_items(const unsigned long long* start, int ntiles, int chun
   if (empty) items[base] = Item{t, 0, s0, s0};
   for (int c = 0; c < cn; ++c, ++k) {
     const long long b = s0 + (long long)c * chunk;
-    items[base + k] = Item{t, 0, b, min(b + (long long)chunk, s1)};
+    items[base + k] = Item{t, k << 8, b, min(b + (long long)chunk, s1)};
   }
   for (int c = 0; c < cw; ++c, ++k) {
     const long long b = s1 + (long long)c * chunk_wide;
-    items[base + k] = Item{t, 1, b, min(b + (long long)chunk_wide, s2)};
+    items[base + k] = Item{t, 1 | (k << 8), b, min(b + (long long)chunk_wide, s2)};
   }
 }
 
@@ -370,7 +391,7 @@ __device__ __forceinline__ float glb_load(float* a) {
 // lie in this tile (and in the grid), w*: the weights per axis.  The terms are
 // the reference's FP32 products ((m*wx)*wy)*wz.  Deterministic mode scales the
 // mass by the power-of-two fixed-point scale first (exact), converts every
-// term to an integer and gives the centre cell the remainder Q(m) - (sum of
+// term to the nearest integer and gives the centre cell the remainder Q(m) - (sum of
 // the other 26), so each particle deposits exactly Q(m) on a periodic grid.
 template <int RED, int TX, int TY, int TZ>
 __device__ __forceinline__ void deposit27(typename AccT<RED>::T* acc, float m, const int lx[3],
@@ -403,13 +424,13 @@ __device__ __forceinline__ void deposit27(typename AccT<RED>::T* acc, float m, c
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           if (a == 1 && b == 1 && c == 1) continue;
-          const int q = (int)(mxy[b][a] * wz[c]);
+          const int q = __float2int_rn(mxy[b][a] * wz[c]);
           rest += q;
           if ((vx >> a) & (vy >> b) & (vz >> c) & 1u)
             lds_add(reinterpret_cast<long long*>(acc) + (lx[a] + TX * (ly[b] + TY * lz[c])), (long long)q);
         }
     if ((vx >> 1) & (vy >> 1) & (vz >> 1) & 1u)
-      lds_add(reinterpret_cast<long long*>(acc) + (lx[1] + TX * (ly[1] + TY * lz[1])), (long long)((int)ms - rest));
+      lds_add(reinterpret_cast<long long*>(acc) + (lx[1] + TX * (ly[1] + TY * lz[1])), (long long)(__float2int_rn(ms) - rest));
   }
 }
 
@@ -457,9 +478,16 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
     __syncthreads();
 
     if (!(it.flags & 1)) {
-      // ---- narrow entries: one lane per entry, 27 cells ----
-      for (long long e = it.begin + tid; e < it.end; e += nthr) {
-        const int4 v = D.entries[e];
+      // ---- narrow entries: one lane per entry, 27 cells; two entries per
+      // lane per pass with both loads in flight before the first is used ----
+      for (long long e0 = it.begin + tid; e0 < it.end; e0 += 2 * nthr) {
+        const long long e1 = e0 + nthr;
+        const int4 v0 = D.entries[e0];
+        const int4 v1 = e1 < it.end ? D.entries[e1] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+        if (u == 1 && e1 >= it.end) break;
+        const int4 v = u == 0 ? v0 : v1;
         int ci[3];
         float wx[3], wy[3], wz[3], m;
         if (MODE == 2) {
@@ -507,6 +535,7 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
           }
         }
         deposit27<RED, TX, TY, TZ>(acc, m, lx, ly, lz, vmask[0], vmask[1], vmask[2], wx, wy, wz, scale);
+        }
       }
     } else if (MODE == 1) {
       // ---- wide table-mode entries: one wave per entry, lanes over cells ----
@@ -528,22 +557,22 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
           const float ms = RED == 0 ? m * scale : m;
           const float term = ((ms * K[ii - ci[0]]) * K[jj - ci[1]]) * K[kk - ci[2]];
           Acc* dst = acc + ((ii - i0) + TX * ((jj - j0) + TY * (kk - k0c)));
-          if (RED == 0) lds_add(reinterpret_cast<long long*>(dst), (long long)(int)term);
+          if (RED == 0) lds_add(reinterpret_cast<long long*>(dst), (long long)__float2int_rn(term));
           else lds_add(reinterpret_cast<float*>(dst), term);
         }
       }
     }
     __syncthreads();
 
-    // ---- merge the chunks of a split tile; the last one writes ----
+    // ---- split tile: every item stores its partial tile; the last one to
+    // finish sums the partials in item order (fixed order: deterministic in
+    // both reductions) and writes the cells ----
     const int nit = D.nitems_tile[t];
     bool writer = true;
     if (nit > 1) {
-      Acc* scr = reinterpret_cast<Acc*>(D.scratch) + (long long)D.split_slot[t] * T;
-      for (int c = tid; c < T; c += nthr) {
-        const Acc v = acc[c];
-        if (v != 0) glb_add(scr + c, v);
-      }
+      Acc* part = reinterpret_cast<Acc*>(D.scratch) + (long long)D.split_slot[t] * T;
+      Acc* mine = part + (long long)(it.flags >> 8) * T;
+      for (int c = tid; c < T; c += nthr) mine[c] = acc[c];
       __threadfence();
       __syncthreads();
       if (tid == 0) {
@@ -553,7 +582,12 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
       __syncthreads();
       writer = s_last;
       if (writer) {
-        for (int c = tid; c < T; c += nthr) acc[c] = glb_load(scr + c);
+        __threadfence();
+        for (int c = tid; c < T; c += nthr) {
+          Acc v = 0;
+          for (int k = 0; k < nit; ++k) v += glb_load(part + (long long)k * T + c);
+          acc[c] = v;
+        }
         __syncthreads();
       }
     }
