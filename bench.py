@@ -49,6 +49,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--tune", default="", help="tx,ty,tz,chunk,waves_per_cu: fixed tuning, no autotune")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="only run a device-to-device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -60,6 +63,18 @@ def main() -> int:
 
     import numpy as np
     import torch
+
+    if args.copy_calibration > 0:
+        # a plain copy of known size: the reference point that scales the
+        # FETCH_SIZE / WRITE_SIZE counters into HBM bytes
+        n = int(args.copy_calibration * (1 << 30)) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": 4 * n, "launches": 5}), flush=True)
+        return 0
 
     from asohf_amd import synth
     from asohf_amd.density import DensityBuilder
@@ -78,7 +93,11 @@ def main() -> int:
     u11 = torch.empty(n11, dtype=torch.float32, device=dev)
     kw = dict(mode=args.mode, reduction=args.reduction, u1=u1, u11=u11)
     tune = bld.get_tune()
-    if not args.no_autotune:
+    if args.tune:
+        tx, ty, tz, ch, wv = (int(v) for v in args.tune.split(","))
+        bld.set_tune(tile=(tx, ty, tz), chunk=ch, waves_per_cu=wv)
+        tune = bld.get_tune()
+    elif not args.no_autotune:
         t0 = time.time()
         tune = bld.autotune(g, x, y, z, m, snap.npart_esp, mode=args.mode, reduction=args.reduction)
         log(f"[bench] autotune picked {tune} in {time.time() - t0:.1f}s")
