@@ -141,7 +141,8 @@ struct asohf_ctx {
   HostPlan plan;
   DBuf plan_blob;
   DBuf cnt, start, cursor, bsum, counters, done, nitems, split, tmass, entries, items, scratch;
-  DBuf flags, idx;
+  DBuf flags, idx, pool, pool_off, pool_nk;
+  long long pool_want = 0;   // pairs the last count pass needed
   DBuf h_in[4], h_out[2], h_mass;    // device copies for the host entry point
   Counters* hcount = nullptr;          // pinned
   unsigned long long* hscalar = nullptr;
@@ -479,7 +480,7 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   D.kmax = 1 << nl;
   D.kw = 2 * D.kmax + 1;
   D.hmask = (unsigned)(cap - 1);
-  for (int a = 0; a < 3; ++a) D.tile[a] = T[a];
+  for (int a = 0; a < 3; ++a) { D.tile[a] = T[a]; D.tshift[a] = __builtin_ctz(T[a]); }
   D.mode = mode;
   // are the centre tables MALLA's recurrence?  (always, unless the caller
   // passed its own COMMON /GRID/ values that differ)
@@ -577,7 +578,19 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   HIPCHK(launch_fill_u32((unsigned*)C, 0u, sizeof(Counters) / 4, s), "clearing counters");
   HIPCHK(launch_fill_u32(ctx->cnt.as<unsigned>(), 0u, nkeys, s), "clearing counts");
   HIPCHK(launch_fill_u32(ctx->done.as<unsigned>(), 0u, ntiles, s), "clearing tile state");
-  HIPCHK(launch_count(P, A, ctx->cnt.as<unsigned>(), C, ctx->cus, s), "count kernel");
+  // saved per-block tables: sized from the previous call (a block that does
+  // not fit recounts in the scatter pass)
+  const long long nblk = bin_blocks(A.n);
+  const long long pool_cap = std::max<long long>(ctx->pool_want + ctx->pool_want / 8, 256 * nblk);
+  HIPCHK(ctx->pool.ensure(sizeof(int2) * pool_cap), "allocating block tables");
+  HIPCHK(ctx->pool_off.ensure(sizeof(long long) * std::max<long long>(nblk, 1)), "allocating block tables");
+  HIPCHK(ctx->pool_nk.ensure(sizeof(int) * std::max<long long>(nblk, 1)), "allocating block tables");
+  BlockTables BT;
+  BT.pool = ctx->pool.as<int2>();
+  BT.cap = (long long)(ctx->pool.cap / sizeof(int2));
+  BT.off = ctx->pool_off.as<long long>();
+  BT.nkeys = ctx->pool_nk.as<int>();
+  HIPCHK(launch_count(P, A, ctx->cnt.as<unsigned>(), C, BT, ctx->cus, s), "count kernel");
   HIPCHK(launch_scan(ctx->cnt.as<unsigned>(), nkeys, ctx->bsum.as<unsigned long long>(),
                      ctx->start.as<unsigned long long>(), ctx->cursor.as<unsigned long long>(), s),
          "scan");
@@ -608,7 +621,8 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   HIPCHK(ctx->scratch.ensure((size_t)std::max<unsigned>(hc.n_split, 1) * T * (red == 0 ? 8 : 4)),
          "allocating split-tile scratch");
   HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
-  HIPCHK(launch_scatter(P, A, ctx->cursor.as<unsigned long long>(), ctx->entries.as<int4>(), n_entries,
+  ctx->pool_want = (long long)hc.pool_fill;
+  HIPCHK(launch_scatter(P, A, ctx->cursor.as<unsigned long long>(), ctx->entries.as<int4>(), n_entries, BT,
                         ctx->cus, s),
          "scatter kernel");
   HIPCHK(hipEventRecord(ctx->ev[3], s), "event");
@@ -744,7 +758,7 @@ void asohf_hip_destroy(asohf_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  DBuf* bufs[] = {&ctx->plan_blob, &ctx->cnt, &ctx->start, &ctx->cursor, &ctx->bsum, &ctx->counters,
+  DBuf* bufs[] = {&ctx->pool, &ctx->pool_off, &ctx->pool_nk, &ctx->plan_blob, &ctx->cnt, &ctx->start, &ctx->cursor, &ctx->bsum, &ctx->counters,
                   &ctx->done, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries, &ctx->items,
                   &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1], &ctx->h_in[2],
                   &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass};

@@ -110,8 +110,8 @@ __device__ __forceinline__ void unpack_table(int4 e, int& ix, int& jy, int& kz, 
 
 // Tiles touched by the wrapped cells {c-1, c, c+1} along one base axis.
 // Returns the number of distinct tile indices (1..3) in t[].
-__device__ __forceinline__ int base_axis_tiles(int c, int n, int T, int t[3]) {
-  int a = (wrap1(c - 1, n) - 1) / T, b = (wrap1(c, n) - 1) / T, d = (wrap1(c + 1, n) - 1) / T;
+__device__ __forceinline__ int base_axis_tiles(int c, int n, int sh, int t[3]) {
+  int a = (wrap1(c - 1, n) - 1) >> sh, b = (wrap1(c, n) - 1) >> sh, d = (wrap1(c + 1, n) - 1) >> sh;
   int k = 0;
   t[k++] = a;
   if (b != a) t[k++] = b;
@@ -151,9 +151,9 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
   member(0, ix, jy, kz);
   {
     int tx[3], ty[3], tz[3];
-    const int nx_ = base_axis_tiles(ix, B.n[0], P.tile[0], tx);
-    const int ny_ = base_axis_tiles(jy, B.n[1], P.tile[1], ty);
-    const int nz_ = base_axis_tiles(kz, B.n[2], P.tile[2], tz);
+    const int nx_ = base_axis_tiles(ix, B.n[0], P.tshift[0], tx);
+    const int ny_ = base_axis_tiles(jy, B.n[1], P.tshift[1], ty);
+    const int nz_ = base_axis_tiles(kz, B.n[2], P.tshift[2], tz);
     int4 e;
     if (MODE == 2) e = make_int4(__float_as_int(x), __float_as_int(y), __float_as_int(z), __float_as_int(m));
     else e = pack_table(ix, jy, kz, 0, m);
@@ -199,9 +199,9 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
       if (MODE == 2) e = make_int4(__float_as_int(x), __float_as_int(y), __float_as_int(z), __float_as_int(m));
       else e = pack_table(ci[0], ci[1], ci[2], irk, m);
       const int wide = (MODE == 1 && irk > 0) ? 1 : 0;
-      const int t0x = (lo[0] - 1) / P.tile[0], t1x = (hi[0] - 1) / P.tile[0];
-      const int t0y = (lo[1] - 1) / P.tile[1], t1y = (hi[1] - 1) / P.tile[1];
-      const int t0z = (lo[2] - 1) / P.tile[2], t1z = (hi[2] - 1) / P.tile[2];
+      const int t0x = (lo[0] - 1) >> P.tshift[0], t1x = (hi[0] - 1) >> P.tshift[0];
+      const int t0y = (lo[1] - 1) >> P.tshift[1], t1y = (hi[1] - 1) >> P.tshift[1];
+      const int t0z = (lo[2] - 1) >> P.tshift[2], t1z = (hi[2] - 1) >> P.tshift[2];
       for (int c = t0z; c <= t1z; ++c)
         for (int b = t0y; b <= t1y; ++b)
           for (int a = t0x; a <= t1x; ++a)

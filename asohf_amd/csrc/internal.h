@@ -56,7 +56,8 @@ struct Plan {
   int kmax;                  // 2^nl
   int kw;                    // 2*kmax + 1
   unsigned hmask;
-  int tile[3];               // tile shape
+  int tile[3];               // tile shape (powers of two)
+  int tshift[3];             // log2 of the tile shape
   int mode;                  // 1 table, 2 TSC
   LevelHash lh[kNLevels + 1];
   float xl;                  // -LADO0/2
@@ -88,6 +89,7 @@ struct Counters {
   unsigned int n_items;
   unsigned int n_split;
   unsigned int n_items_w;    // second planning pass: items written so far
+  unsigned long long pool_fill;  // saved block tables: pairs used so far
   double scale;              // fixed-point scale (power of two)
   double inv_scale;
 };
@@ -98,6 +100,15 @@ struct Item {
   int flags;                 // bit 0: wide (table-mode N >= 2) entries
   long long begin;           // entry range [begin, end)
   long long end;
+};
+
+// Per-workgroup (key, count) tables saved by the count pass for the scatter
+// pass, in a compact pool; off = -1 when the table did not fit.
+struct BlockTables {
+  int2* pool;                // (key, count) pairs
+  long long cap;             // pairs the pool holds
+  long long* off;            // [blocks] start of each block's pairs, or -1
+  int* nkeys;                // [blocks]
 };
 
 // Arguments of the persistent deposit.

@@ -119,7 +119,8 @@ struct Prefetch {
 };
 
 template <int MODE>
-__global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, unsigned* cnt, Counters* C) {
+__global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, unsigned* cnt, Counters* C,
+                                                            BlockTables BT) {
   __shared__ int keys[kBinCap];
   __shared__ unsigned vals[kBinCap];
   for (int k = threadIdx.x; k < kBinCap; k += kBinThreads) { keys[k] = -1; vals[k] = 0u; }
@@ -149,8 +150,28 @@ __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, uns
         [&](int, int, int, int) { ++pairs; }, err);
   }
   __syncthreads();
+  // flush the counts, and save the table (compacted) for the scatter pass
+  __shared__ int s_n;
+  __shared__ long long s_off;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
   for (int k = threadIdx.x; k < kBinCap; k += kBinThreads)
-    if (keys[k] >= 0) atomicAdd(&cnt[keys[k]], vals[k]);
+    if (keys[k] >= 0) {
+      atomicAdd(&cnt[keys[k]], vals[k]);
+      atomicAdd(&s_n, 1);
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const long long off = (long long)atomicAdd(&C->pool_fill, (unsigned long long)s_n);
+    s_off = off + s_n <= BT.cap ? off : -1;
+    BT.off[blockIdx.x] = s_off;
+    BT.nkeys[blockIdx.x] = s_n;
+    s_n = 0;
+  }
+  __syncthreads();
+  if (s_off >= 0)
+    for (int k = threadIdx.x; k < kBinCap; k += kBinThreads)
+      if (keys[k] >= 0) BT.pool[s_off + atomicAdd(&s_n, 1)] = make_int2(keys[k], (int)vals[k]);
   pairs = wave_sum(pairs);
   mmax = wave_max(mmax);
   err = wave_or(err);
@@ -163,7 +184,8 @@ __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, uns
 
 template <int MODE>
 __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, unsigned long long* cursor,
-                                                              int4* entries, unsigned long long cap) {
+                                                              int4* entries, unsigned long long cap,
+                                                              BlockTables BT) {
   __shared__ int keys[kBinCap];
   __shared__ unsigned vals[kBinCap];
   __shared__ unsigned long long base[kBinCap];
@@ -171,22 +193,33 @@ __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, u
   __syncthreads();
   unsigned err = 0;
   const long long c0 = (long long)blockIdx.x * kBinChunk;
-  // phase A: count this block's emissions per key
-  Prefetch pf(A, c0 + threadIdx.x);
-  for (int r = 0; r < kBinPer; ++r) {
-    const long long i = c0 + (long long)r * kBinThreads + threadIdx.x;
-    if (i >= A.n) break;
-    float x, y, z, m;
-    pf.next(A, i + kBinThreads, r + 1 < kBinPer, x, y, z, m);
-    if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m)) || m < 0.f) continue;
-    const int s = MODE == 1 ? species_of(A, i) : 0;
-    enumerate_particle<MODE>(
-        P, i, x, y, z, m, s,
-        [&](int, int t, int wide, int4) {
-          const int h = lds_insert(keys, 2 * t + wide);
-          if (h >= 0) atomicAdd(&vals[h], 1u);
-        },
-        [&](int, int, int, int) {}, err);
+  const long long saved = BT.off[blockIdx.x];
+  if (saved >= 0) {
+    // the count pass's table of this block: reinsert its keys
+    const int nk = BT.nkeys[blockIdx.x];
+    for (int j = threadIdx.x; j < nk; j += kBinThreads) {
+      const int2 kv = BT.pool[saved + j];
+      const int h = lds_insert(keys, kv.x);
+      if (h >= 0) vals[h] = (unsigned)kv.y;
+    }
+  } else {
+    // phase A: count this block's emissions per key
+    Prefetch pf(A, c0 + threadIdx.x);
+    for (int r = 0; r < kBinPer; ++r) {
+      const long long i = c0 + (long long)r * kBinThreads + threadIdx.x;
+      if (i >= A.n) break;
+      float x, y, z, m;
+      pf.next(A, i + kBinThreads, r + 1 < kBinPer, x, y, z, m);
+      if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m)) || m < 0.f) continue;
+      const int s = MODE == 1 ? species_of(A, i) : 0;
+      enumerate_particle<MODE>(
+          P, i, x, y, z, m, s,
+          [&](int, int t, int wide, int4) {
+            const int h = lds_insert(keys, 2 * t + wide);
+            if (h >= 0) atomicAdd(&vals[h], 1u);
+          },
+          [&](int, int, int, int) {}, err);
+    }
   }
   __syncthreads();
   // phase B: one global reservation per key
@@ -480,13 +513,19 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
     if (!(it.flags & 1)) {
       // ---- narrow entries: one lane per entry, 27 cells; two entries per
       // lane per pass with both loads in flight before the first is used ----
-      for (long long e0 = it.begin + tid; e0 < it.end; e0 += 2 * nthr) {
-        const long long e1 = e0 + nthr;
+      // each lane walks its own contiguous stretch of the item, so the 64
+      // lanes of a wave hold particles from across the tile's list rather
+      // than 64 neighbours (fewer LDS address and bank conflicts)
+      const long long len = it.end - it.begin;
+      const long long per = (len + nthr - 1) / nthr;
+      const long long lane_end = min(it.begin + (long long)(tid + 1) * per, it.end);
+      for (long long e0 = it.begin + (long long)tid * per; e0 < lane_end; e0 += 2) {
+        const long long e1 = e0 + 1;
         const int4 v0 = D.entries[e0];
-        const int4 v1 = e1 < it.end ? D.entries[e1] : make_int4(0, 0, 0, 0);
+        const int4 v1 = e1 < lane_end ? D.entries[e1] : make_int4(0, 0, 0, 0);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-        if (u == 1 && e1 >= it.end) break;
+        if (u == 1 && e1 >= lane_end) break;
         const int4 v = u == 0 ? v0 : v1;
         int ci[3];
         float wx[3], wy[3], wz[3], m;
@@ -937,22 +976,25 @@ hipError_t launch_fill_u32(unsigned* p, unsigned v, long long n, hipStream_t s) 
   return hipGetLastError();
 }
 
-hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* C, int cus, hipStream_t s) {
+long long bin_blocks(long long n) { return (n + kBinChunk - 1) / kBinChunk; }
+
+hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* C, const BlockTables& BT,
+                        int cus, hipStream_t s) {
   (void)cus;
   if (A.n <= 0) return hipSuccess;
-  const int grid = (int)((A.n + kBinChunk - 1) / kBinChunk);
-  if (P.mode == 2) count_kernel<2><<<grid, 256, 0, s>>>(P, A, cnt, C);
-  else count_kernel<1><<<grid, 256, 0, s>>>(P, A, cnt, C);
+  const int grid = (int)bin_blocks(A.n);
+  if (P.mode == 2) count_kernel<2><<<grid, 256, 0, s>>>(P, A, cnt, C, BT);
+  else count_kernel<1><<<grid, 256, 0, s>>>(P, A, cnt, C, BT);
   return hipGetLastError();
 }
 
 hipError_t launch_scatter(const Plan& P, const Parts& A, unsigned long long* cursor, int4* entries,
-                          unsigned long long cap, int cus, hipStream_t s) {
+                          unsigned long long cap, const BlockTables& BT, int cus, hipStream_t s) {
   (void)cus;
   if (A.n <= 0) return hipSuccess;
-  const int grid = (int)((A.n + kBinChunk - 1) / kBinChunk);
-  if (P.mode == 2) scatter_kernel<2><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap);
-  else scatter_kernel<1><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap);
+  const int grid = (int)bin_blocks(A.n);
+  if (P.mode == 2) scatter_kernel<2><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap, BT);
+  else scatter_kernel<1><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap, BT);
   return hipGetLastError();
 }
 

@@ -7,9 +7,11 @@
 namespace asohf {
 
 hipError_t launch_fill_u32(unsigned* p, unsigned v, long long n, hipStream_t s);
-hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* C, int cus, hipStream_t s);
+long long bin_blocks(long long n);
+hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* C, const BlockTables& BT,
+                        int cus, hipStream_t s);
 hipError_t launch_scatter(const Plan& P, const Parts& A, unsigned long long* cursor, int4* entries,
-                          unsigned long long cap, int cus, hipStream_t s);
+                          unsigned long long cap, const BlockTables& BT, int cus, hipStream_t s);
 hipError_t launch_membership(const Plan& P, const Parts& A, int* out, long long cap,
                              unsigned long long* count, int cus, hipStream_t s);
 long long scan_scratch_elems(long long n);
