@@ -141,7 +141,7 @@ struct asohf_ctx {
   HostPlan plan;
   DBuf plan_blob;
   DBuf cnt, start, cursor, bsum, counters, done, nitems, split, tmass, entries, items, scratch;
-  DBuf flags, idx, pool, pool_off, pool_nk;
+  DBuf flags, idx, pool, pool_off, pool_nk, blk_pairs, blk_mmax, nsplit, item_off, split_off;
   long long pool_want = 0;   // pairs the last count pass needed
   DBuf h_in[4], h_out[2], h_mass;    // device copies for the host entry point
   Counters* hcount = nullptr;          // pinned
@@ -563,51 +563,71 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   const long long nkeys = 2ll * ntiles;
   const int T = P.tile[0] * P.tile[1] * P.tile[2];
   const int red = o.reduction;
+  if (A.n > (1ll << 32)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^32 particles");
+  const long long nblk = bin_blocks(A.n);
   HIPCHK(ctx->cnt.ensure(sizeof(unsigned) * std::max<long long>(nkeys, 1)), "allocating counts");
   HIPCHK(ctx->start.ensure(sizeof(unsigned long long) * (nkeys + 1)), "allocating offsets");
   HIPCHK(ctx->cursor.ensure(sizeof(unsigned long long) * std::max<long long>(nkeys, 1)), "allocating cursors");
   HIPCHK(ctx->bsum.ensure(sizeof(unsigned long long) * scan_scratch_elems(nkeys)), "allocating scan");
   HIPCHK(ctx->counters.ensure(sizeof(Counters)), "allocating counters");
   HIPCHK(ctx->done.ensure(sizeof(unsigned) * std::max(ntiles, 1)), "allocating tile state");
-  HIPCHK(ctx->nitems.ensure(sizeof(int) * std::max(ntiles, 1)), "allocating tile state");
+  HIPCHK(ctx->nitems.ensure(sizeof(unsigned) * std::max(ntiles, 1)), "allocating tile state");
+  HIPCHK(ctx->nsplit.ensure(sizeof(unsigned) * std::max(ntiles, 1)), "allocating tile state");
+  HIPCHK(ctx->item_off.ensure(sizeof(unsigned long long) * (ntiles + 1)), "allocating tile state");
+  HIPCHK(ctx->split_off.ensure(sizeof(unsigned long long) * (ntiles + 1)), "allocating tile state");
   HIPCHK(ctx->split.ensure(sizeof(int) * std::max(ntiles, 1)), "allocating tile state");
   HIPCHK(ctx->tmass.ensure(sizeof(double) * std::max(ntiles, 1)), "allocating tile sums");
-  Counters* C = ctx->counters.as<Counters>();
-
-  HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
-  HIPCHK(launch_fill_u32((unsigned*)C, 0u, sizeof(Counters) / 4, s), "clearing counters");
-  HIPCHK(launch_fill_u32(ctx->cnt.as<unsigned>(), 0u, nkeys, s), "clearing counts");
-  HIPCHK(launch_fill_u32(ctx->done.as<unsigned>(), 0u, ntiles, s), "clearing tile state");
-  // saved per-block tables: sized from the previous call (a block that does
-  // not fit recounts in the scatter pass)
-  const long long nblk = bin_blocks(A.n);
+  // saved per-block tables: sized from the previous call (a block whose table
+  // does not fit its sub-pool recounts in the scatter pass)
   const long long pool_cap = std::max<long long>(ctx->pool_want + ctx->pool_want / 8, 256 * nblk);
   HIPCHK(ctx->pool.ensure(sizeof(int2) * pool_cap), "allocating block tables");
   HIPCHK(ctx->pool_off.ensure(sizeof(long long) * std::max<long long>(nblk, 1)), "allocating block tables");
   HIPCHK(ctx->pool_nk.ensure(sizeof(int) * std::max<long long>(nblk, 1)), "allocating block tables");
+  HIPCHK(ctx->blk_pairs.ensure(sizeof(unsigned long long) * std::max<long long>(nblk, 1)), "allocating block sums");
+  HIPCHK(ctx->blk_mmax.ensure(sizeof(unsigned) * std::max<long long>(nblk, 1)), "allocating block sums");
   BlockTables BT;
   BT.pool = ctx->pool.as<int2>();
   BT.cap = (long long)(ctx->pool.cap / sizeof(int2));
   BT.off = ctx->pool_off.as<long long>();
   BT.nkeys = ctx->pool_nk.as<int>();
-  HIPCHK(launch_count(P, A, ctx->cnt.as<unsigned>(), C, BT, ctx->cus, s), "count kernel");
-  HIPCHK(launch_scan(ctx->cnt.as<unsigned>(), nkeys, ctx->bsum.as<unsigned long long>(),
-                     ctx->start.as<unsigned long long>(), ctx->cursor.as<unsigned long long>(), s),
-         "scan");
+  BT.pairs = ctx->blk_pairs.as<unsigned long long>();
+  BT.mmax = ctx->blk_mmax.as<unsigned>();
+  Counters* C = ctx->counters.as<Counters>();
   const int chunk = std::max(ctx->tune.chunk, 64);
   const int chunk_wide = std::max(chunk / 64, 4);
-  HIPCHK(launch_plan_items(ctx->start.as<unsigned long long>(), ntiles, chunk, chunk_wide, 0, nullptr,
-                           ctx->nitems.as<int>(), ctx->split.as<int>(), C, s),
-         "planning items");
+  unsigned long long* start = ctx->start.as<unsigned long long>();
+  unsigned long long* bsum = ctx->bsum.as<unsigned long long>();
+
+  HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
+  HIPCHK(launch_fill_u32((unsigned*)C, 0u, sizeof(Counters) / 4, s), "clearing counters");
+  HIPCHK(launch_fill_u32(ctx->cnt.as<unsigned>(), 0u, nkeys, s), "clearing counts");
+  HIPCHK(launch_fill_u32(ctx->done.as<unsigned>(), 0u, ntiles, s), "clearing tile state");
+  HIPCHK(launch_count(P, A, ctx->cnt.as<unsigned>(), C, BT, ctx->cus, s), "count kernel");
+  if (nblk > 0) HIPCHK(launch_reduce_blocks(BT, nblk, C, s), "block sums");
+  HIPCHK(launch_scan(ctx->cnt.as<unsigned>(), nkeys, bsum, start, ctx->cursor.as<unsigned long long>(), s),
+         "scan");
+  HIPCHK(launch_plan_count(start, ntiles, chunk, chunk_wide, ctx->nitems.as<unsigned>(),
+                           ctx->nsplit.as<unsigned>(), s), "planning items");
+  HIPCHK(launch_scan(ctx->nitems.as<unsigned>(), ntiles, bsum, ctx->item_off.as<unsigned long long>(), nullptr, s),
+         "item offsets");
+  HIPCHK(launch_scan(ctx->nsplit.as<unsigned>(), ntiles, bsum, ctx->split_off.as<unsigned long long>(), nullptr, s),
+         "split offsets");
   HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
   HIPCHK(hipMemcpyAsync(ctx->hcount, C, sizeof(Counters), hipMemcpyDeviceToHost, s), "reading counters");
-  HIPCHK(hipMemcpyAsync(ctx->hscalar, ctx->start.as<unsigned long long>() + nkeys, sizeof(unsigned long long),
-                        hipMemcpyDeviceToHost, s), "reading totals");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar, start + nkeys, sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
+         "reading totals");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar + 1, ctx->item_off.as<unsigned long long>() + ntiles,
+                        sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "reading totals");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar + 2, ctx->split_off.as<unsigned long long>() + ntiles,
+                        sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "reading totals");
   HIPCHK(hipStreamSynchronize(s), "count pass");
-  const Counters hc = *ctx->hcount;
-  const unsigned long long n_entries = *ctx->hscalar;
+  const Counters& hc = *ctx->hcount;
+  const unsigned long long n_entries = ctx->hscalar[0];
+  const unsigned long long n_items = ctx->hscalar[1];
+  const unsigned long long n_split = ctx->hscalar[2];
   asohf_status st = decode_err(ctx, hc.err);
   if (st != ASOHF_OK) return st;
+  if (n_items >= (1ull << 31)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^31 work items");
   // fixed-point scale: the power of two with max(m) * scale in [2^30, 2^31),
   // so every quantised term fits an int32 and any sum over <= 2^32 particles
   // an int64
@@ -615,19 +635,20 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   int e2 = 0;
   if (mmax > 0.f) (void)std::frexp(mmax, &e2);
   const double scale = std::ldexp(1.0, 31 - e2);
-  if (A.n > (1ll << 32)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^32 particles");
+  unsigned long long sub_max = 0;
+  for (int k = 0; k < kPoolSubs; ++k) sub_max = std::max(sub_max, hc.pool_fill[k * (kLine / 2)]);
+  ctx->pool_want = (long long)(sub_max * kPoolSubs);
   HIPCHK(ctx->entries.ensure(sizeof(int4) * std::max<unsigned long long>(n_entries, 1)), "allocating entries");
-  HIPCHK(ctx->items.ensure(sizeof(Item) * std::max<unsigned>(hc.n_items, 1)), "allocating work items");
-  HIPCHK(ctx->scratch.ensure((size_t)std::max<unsigned>(hc.n_split, 1) * T * (red == 0 ? 8 : 4)),
+  HIPCHK(ctx->items.ensure(sizeof(Item) * std::max<unsigned long long>(n_items, 1)), "allocating work items");
+  HIPCHK(ctx->scratch.ensure((size_t)std::max<unsigned long long>(n_split, 1) * T * (red == 0 ? 8 : 4)),
          "allocating split-tile scratch");
   HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
-  ctx->pool_want = (long long)hc.pool_fill;
   HIPCHK(launch_scatter(P, A, ctx->cursor.as<unsigned long long>(), ctx->entries.as<int4>(), n_entries, BT,
                         ctx->cus, s),
          "scatter kernel");
   HIPCHK(hipEventRecord(ctx->ev[3], s), "event");
-  HIPCHK(launch_plan_items(ctx->start.as<unsigned long long>(), ntiles, chunk, chunk_wide, 1,
-                           ctx->items.as<Item>(), ctx->nitems.as<int>(), ctx->split.as<int>(), C, s),
+  HIPCHK(launch_plan_write(start, ntiles, chunk, chunk_wide, ctx->item_off.as<unsigned long long>(),
+                           ctx->split_off.as<unsigned long long>(), ctx->items.as<Item>(), ctx->split.as<int>(), s),
          "writing items");
   DepArgs D;
   memset(&D, 0, sizeof(D));
@@ -644,10 +665,12 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   D.tile_mass_d = ctx->tmass.as<double>();
   D.scale = scale;
   D.inv_scale = 1.0 / scale;
+  D.n_items = (unsigned)n_items;
   const int waves = std::max(ctx->tune.waves_per_cu, 1);
   const int block = ctx->tune.block;
   int grid = std::max(1, ctx->cus * waves / std::max(block / 64, 1));
-  grid = (int)std::min<long long>(grid, std::max<unsigned>(hc.n_items, 1));
+  grid = (int)std::min<long long>(grid, std::max<unsigned long long>(n_items, 1));
+  D.n_queues = std::min(kQueues, grid);
   HIPCHK(hipEventRecord(ctx->ev[4], s), "event");
   HIPCHK(launch_deposit(P, D, red, grid, block, s), "deposit kernel");
   HIPCHK(hipEventRecord(ctx->ev[5], s), "event");
@@ -658,7 +681,7 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   ctx->stats.n_pairs = (int64_t)hc.n_pairs;
   ctx->stats.n_entries = (int64_t)n_entries;
   ctx->stats.n_tiles = ntiles;
-  ctx->stats.n_items = hc.n_items;
+  ctx->stats.n_items = (int64_t)n_items;
   ctx->stats.n_cells = H.ncells;
   ctx->stats_pending = true;
   ctx->stats_phases = true;
@@ -758,7 +781,8 @@ void asohf_hip_destroy(asohf_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-  DBuf* bufs[] = {&ctx->pool, &ctx->pool_off, &ctx->pool_nk, &ctx->plan_blob, &ctx->cnt, &ctx->start, &ctx->cursor, &ctx->bsum, &ctx->counters,
+  DBuf* bufs[] = {&ctx->pool, &ctx->pool_off, &ctx->pool_nk, &ctx->blk_pairs, &ctx->blk_mmax, &ctx->nsplit,
+                  &ctx->item_off, &ctx->split_off, &ctx->plan_blob, &ctx->cnt, &ctx->start, &ctx->cursor, &ctx->bsum, &ctx->counters,
                   &ctx->done, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries, &ctx->items,
                   &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1], &ctx->h_in[2],
                   &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass};

@@ -79,19 +79,18 @@ struct Parts {
 };
 
 // Device counters of one call (one small buffer, cleared by a fill kernel).
+// Counters that many workgroups update sit on their own 128-byte lines.
+constexpr int kPoolSubs = 64;       // sub-pools of the saved block tables
+constexpr int kQueues = 8;          // deposit work queues (one per XCD)
+constexpr int kLine = 32;           // 128 bytes in 4-byte units
 struct Counters {
-  unsigned long long n_pairs;
-  unsigned long long n_entries;
-  unsigned int err;          // bit 0: non-finite position/mass, bit 1: base index outside
-                             // the single-wrap range, bit 2: negative mass
-  unsigned int mmax_bits;    // max particle mass, as FP32 bits (masses >= 0)
-  unsigned int queue_head;   // persistent work queue
-  unsigned int n_items;
-  unsigned int n_split;
-  unsigned int n_items_w;    // second planning pass: items written so far
-  unsigned long long pool_fill;  // saved block tables: pairs used so far
-  double scale;              // fixed-point scale (power of two)
-  double inv_scale;
+  unsigned long long n_pairs;     // reduced from per-block partials
+  unsigned int err;               // bit 0: non-finite position/mass, bit 1: base index outside
+                                  // the single-wrap range, bit 2: negative mass
+  unsigned int mmax_bits;         // max particle mass, as FP32 bits (masses >= 0)
+  unsigned int pad[kLine - 4];
+  unsigned long long pool_fill[kPoolSubs * kLine / 2];   // sub-pool k at [k * kLine / 2]
+  unsigned int queue[kQueues * kLine];                   // queue q at [q * kLine]
 };
 
 // Work item of the persistent deposit: a chunk of one tile's entry list.
@@ -105,10 +104,12 @@ struct Item {
 // Per-workgroup (key, count) tables saved by the count pass for the scatter
 // pass, in a compact pool; off = -1 when the table did not fit.
 struct BlockTables {
-  int2* pool;                // (key, count) pairs
+  int2* pool;                // (key, count) pairs, kPoolSubs equal sub-pools
   long long cap;             // pairs the pool holds
   long long* off;            // [blocks] start of each block's pairs, or -1
   int* nkeys;                // [blocks]
+  unsigned long long* pairs; // [blocks] (particle, grid) members counted by the block
+  unsigned* mmax;            // [blocks] max mass of the block, FP32 bits
 };
 
 // Arguments of the persistent deposit.
@@ -126,6 +127,8 @@ struct DepArgs {
   double* tile_mass_d;       // atomic: FP64 mass per tile
   double scale;              // fixed-point scale
   double inv_scale;
+  unsigned n_items;          // items in the queue
+  int n_queues;              // queues in use (<= kQueues, <= grid)
 };
 
 enum : int { kErrNonFinite = 1, kErrWrap = 2, kErrNegMass = 4, kErrIndexRange = 8 };

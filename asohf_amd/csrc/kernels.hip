@@ -162,8 +162,11 @@ __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, uns
     }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const long long off = (long long)atomicAdd(&C->pool_fill, (unsigned long long)s_n);
-    s_off = off + s_n <= BT.cap ? off : -1;
+    // sub-pool of this block: 1/kPoolSubs of the pool, its own counter line
+    const int sub = blockIdx.x & (kPoolSubs - 1);
+    const long long subcap = BT.cap / kPoolSubs;
+    const long long off = (long long)atomicAdd(&C->pool_fill[sub * (kLine / 2)], (unsigned long long)s_n);
+    s_off = off + s_n <= subcap ? sub * subcap + off : -1;
     BT.off[blockIdx.x] = s_off;
     BT.nkeys[blockIdx.x] = s_n;
     s_n = 0;
@@ -172,13 +175,44 @@ __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, uns
   if (s_off >= 0)
     for (int k = threadIdx.x; k < kBinCap; k += kBinThreads)
       if (keys[k] >= 0) BT.pool[s_off + atomicAdd(&s_n, 1)] = make_int2(keys[k], (int)vals[k]);
+  // per-block partials (no contended global atomics); reduce_blocks folds them
+  __shared__ unsigned long long s_pairs[kBinThreads / 64];
+  __shared__ float s_mmax[kBinThreads / 64];
   pairs = wave_sum(pairs);
   mmax = wave_max(mmax);
   err = wave_or(err);
   if ((threadIdx.x & 63) == 0) {
-    if (pairs) atomicAdd(&C->n_pairs, pairs);
-    atomicMax(&C->mmax_bits, __float_as_uint(mmax));
+    s_pairs[threadIdx.x >> 6] = pairs;
+    s_mmax[threadIdx.x >> 6] = mmax;
     if (err) atomicOr(&C->err, err);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long p = 0;
+    float mm = 0.f;
+    for (int w = 0; w < kBinThreads / 64; ++w) { p += s_pairs[w]; mm = fmaxf(mm, s_mmax[w]); }
+    BT.pairs[blockIdx.x] = p;
+    BT.mmax[blockIdx.x] = __float_as_uint(mm);
+  }
+}
+
+__global__ void __launch_bounds__(1024) reduce_blocks(BlockTables BT, long long nblk, Counters* C) {
+  __shared__ unsigned long long sp[16];
+  __shared__ float sm[16];
+  unsigned long long p = 0;
+  float mm = 0.f;
+  for (long long b = threadIdx.x; b < nblk; b += blockDim.x) {
+    p += BT.pairs[b];
+    mm = fmaxf(mm, __uint_as_float(BT.mmax[b]));
+  }
+  p = wave_sum(p);
+  mm = wave_max(mm);
+  if ((threadIdx.x & 63) == 0) { sp[threadIdx.x >> 6] = p; sm[threadIdx.x >> 6] = mm; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { p = sp[0] + sp[w]; sp[0] = p; sm[0] = fmaxf(sm[0], sm[w]); }
+    C->n_pairs = sp[0];
+    C->mmax_bits = __float_as_uint(sm[0]);
   }
 }
 
@@ -358,27 +392,41 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply(const unsigned* in, l
 // ---------------------------------------------------------------------------
 // work items
 // ---------------------------------------------------------------------------
-// pass 0: items per tile, split-tile slots and the totals; pass 1: write the items
-__global__ void plan_items(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
-                           int write, Item* items, int* nitems_tile, int* split_slot, Counters* C) {
+// Work items.  plan_count: items per tile and the partial-tile slots a split
+// tile needs; after exclusive scans of both, plan_write places every tile's
+// items at its offset (no atomics: the item order is the tile order).
+__device__ __forceinline__ void tile_chunks(const unsigned long long* start, int t, int chunk, int chunk_wide,
+                                            long long& s0, long long& s1, long long& s2, int& cn, int& cw) {
+  s0 = start[2 * t]; s1 = start[2 * t + 1]; s2 = start[2 * t + 2];
+  cn = (int)((s1 - s0 + chunk - 1) / chunk);
+  cw = (int)((s2 - s1 + chunk_wide - 1) / chunk_wide);
+}
+
+__global__ void plan_count(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
+                           unsigned* nitems_tile, unsigned* nsplit_tile) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
-  const long long s0 = start[2 * t], s1 = start[2 * t + 1], s2 = start[2 * t + 2];
-  const long long nn = s1 - s0, nw = s2 - s1;
-  const int cn = (int)((nn + chunk - 1) / chunk), cw = (int)((nw + chunk_wide - 1) / chunk_wide);
-  int tot = cn + cw;
-  const int empty = tot == 0;
-  if (empty) tot = 1;   // an empty tile still writes its zeros
-  if (!write) {
-    atomicAdd(&C->n_items, (unsigned)tot);
-    nitems_tile[t] = tot;
-    // a split tile owns tot consecutive partial-tile slots
-    split_slot[t] = tot > 1 ? (int)atomicAdd(&C->n_split, (unsigned)tot) : -1;
-    return;
-  }
-  const unsigned base = atomicAdd(&C->n_items_w, (unsigned)tot);
+  long long s0, s1, s2;
+  int cn, cw;
+  tile_chunks(start, t, chunk, chunk_wide, s0, s1, s2, cn, cw);
+  const int tot = max(cn + cw, 1);   // an empty tile still writes its zeros
+  nitems_tile[t] = (unsigned)tot;
+  nsplit_tile[t] = tot > 1 ? (unsigned)tot : 0u;
+}
+
+__global__ void plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
+                           const unsigned long long* item_off, const unsigned long long* split_off,
+                           Item* items, int* split_slot) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  long long s0, s1, s2;
+  int cn, cw;
+  tile_chunks(start, t, chunk, chunk_wide, s0, s1, s2, cn, cw);
+  const long long base = (long long)item_off[t];
+  const int tot = max(cn + cw, 1);
+  split_slot[t] = tot > 1 ? (int)split_off[t] : -1;
+  if (cn + cw == 0) { items[base] = Item{t, 0, s0, s0}; return; }
   int k = 0;
-  if (empty) items[base] = Item{t, 0, s0, s0};
   for (int c = 0; c < cn; ++c, ++k) {
     const long long b = s0 + (long long)c * chunk;
     items[base + k] = Item{t, k << 8, b, min(b + (long long)chunk, s1)};
@@ -387,14 +435,6 @@ __global__ void plan_items(const unsigned long long* start, int ntiles, int chun
     const long long b = s1 + (long long)c * chunk_wide;
     items[base + k] = Item{t, 1 | (k << 8), b, min(b + (long long)chunk_wide, s2)};
   }
-}
-
-template <class T>
-__global__ void zero_split_scratch(T* scr, long long per, const Counters* C) {
-  const long long n = (long long)C->n_split * per;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    scr[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -492,12 +532,15 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
   const float scale = (float)D.scale;
   const float k0[3] = {P.ktab[P.kmax - 1], P.ktab[P.kmax], P.ktab[P.kmax + 1]};
 
+  // workgroup w serves queue w mod n_queues, which hands out items
+  // q, q + n_queues, ...; consecutive workgroups land on different XCDs, so
+  // each queue counter lives in one XCD's L2
+  const int q = blockIdx.x % D.n_queues;
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(&D.C->queue_head, 1u);
+    if (tid == 0) s_item = atomicAdd(&D.C->queue[q * kLine], 1u) * D.n_queues + q;
     __syncthreads();
     const unsigned idx = __builtin_amdgcn_readfirstlane(s_item);
-    const unsigned n_items = D.C->n_items;
-    if (idx >= n_items) break;
+    if (idx >= D.n_items) break;
     const Item it = D.items[idx];
     const int t = __builtin_amdgcn_readfirstlane(it.tile);
     const int g = __builtin_amdgcn_readfirstlane(P.tile_grid[t]);
@@ -1017,19 +1060,23 @@ hipError_t launch_scan(const unsigned* in, long long n, unsigned long long* bsum
   return hipGetLastError();
 }
 
-hipError_t launch_plan_items(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
-                             int write, Item* items, int* nitems_tile, int* split_slot, Counters* C,
-                             hipStream_t s) {
-  plan_items<<<grid_for(ntiles, 256, 1 << 30), 256, 0, s>>>(start, ntiles, chunk, chunk_wide, write, items,
-                                                           nitems_tile, split_slot, C);
+hipError_t launch_reduce_blocks(const BlockTables& BT, long long nblk, Counters* C, hipStream_t s) {
+  reduce_blocks<<<1, 1024, 0, s>>>(BT, nblk, C);
   return hipGetLastError();
 }
 
-hipError_t launch_zero_split(void* scr, int reduction, long long per, long long cap_elems,
-                             const Counters* C, hipStream_t s) {
-  const int grid = grid_for(cap_elems, 256, 8192);
-  if (reduction == 0) zero_split_scratch<long long><<<grid, 256, 0, s>>>((long long*)scr, per, C);
-  else zero_split_scratch<float><<<grid, 256, 0, s>>>((float*)scr, per, C);
+hipError_t launch_plan_count(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
+                             unsigned* nitems_tile, unsigned* nsplit_tile, hipStream_t s) {
+  plan_count<<<grid_for(ntiles, 256, 1 << 30), 256, 0, s>>>(start, ntiles, chunk, chunk_wide, nitems_tile,
+                                                           nsplit_tile);
+  return hipGetLastError();
+}
+
+hipError_t launch_plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
+                             const unsigned long long* item_off, const unsigned long long* split_off,
+                             Item* items, int* split_slot, hipStream_t s) {
+  plan_write<<<grid_for(ntiles, 256, 1 << 30), 256, 0, s>>>(start, ntiles, chunk, chunk_wide, item_off,
+                                                           split_off, items, split_slot);
   return hipGetLastError();
 }
 

@@ -17,11 +17,12 @@ hipError_t launch_membership(const Plan& P, const Parts& A, int* out, long long 
 long long scan_scratch_elems(long long n);
 hipError_t launch_scan(const unsigned* in, long long n, unsigned long long* bsum, unsigned long long* start,
                        unsigned long long* cursor, hipStream_t s);
-hipError_t launch_plan_items(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
-                             int write, Item* items, int* nitems_tile, int* split_slot, Counters* C,
-                             hipStream_t s);
-hipError_t launch_zero_split(void* scr, int reduction, long long per, long long cap_elems,
-                             const Counters* C, hipStream_t s);
+hipError_t launch_reduce_blocks(const BlockTables& BT, long long nblk, Counters* C, hipStream_t s);
+hipError_t launch_plan_count(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
+                             unsigned* nitems_tile, unsigned* nsplit_tile, hipStream_t s);
+hipError_t launch_plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
+                             const unsigned long long* item_off, const unsigned long long* split_off,
+                             Item* items, int* split_slot, hipStream_t s);
 bool tile_shape_supported(const int t[3]);
 hipError_t launch_deposit(const Plan& P, const DepArgs& D, int reduction, int grid, int block, hipStream_t s);
 hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, double* grid_mass, hipStream_t s);
