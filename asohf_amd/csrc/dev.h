@@ -47,13 +47,24 @@ __device__ __forceinline__ void tsc_dweights(float xp, float c0, float c1, float
   t = xp - c2; dw[2] = -(1.5f - fabsf(t) / d) * (t > 0.f ? 1.f : (t < 0.f ? -1.f : 0.f)) / d;
 }
 
+// The quotient a/d whose integer part is a cell index, exact where it
+// matters: t = a*(1/d) is within |q|*1.2e-7 of q = a/d and RN(q) within
+// |q|*6e-8, so when t is farther than |t|*2^-20 from every integer, RN(q) lies
+// strictly between the same two integers and truncating or flooring either
+// gives the same index; otherwise the IEEE division is taken.
+__device__ __forceinline__ float index_quotient(float a, float d, float inv_d) {
+  const float t = a * inv_d;
+  if (fabsf(t - rintf(t)) > fabsf(t) * 9.5367431640625e-7f) return t;
+  return a / d;
+}
+
 // MINIRADX(II) = PATCHRX + (FLOAT(II)-1.5)*DXPA  (grids.f:1304-1312)
 __device__ __forceinline__ float minirad(float pr, int ii, float d) { return pr + ((float)ii - 1.5f) * d; }
 
 // INT((XP-XL)/DX)+1 clamped, on the base grid; t is guarded so the
 // conversion is always defined.
-__device__ __forceinline__ int tsc_base_index(float xp, float xl, float d, int n) {
-  float t = (xp - xl) / d;
+__device__ __forceinline__ int tsc_base_index(float xp, float xl, float d, float inv_d, int n) {
+  float t = index_quotient(xp - xl, d, inv_d);
   if (!(t < 1.0e9f)) t = 1.0e9f;
   if (!(t > -1.0e9f)) t = -1.0e9f;
   return clampi((int)t + 1, 1, n);
@@ -132,11 +143,13 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
   // ---------------- base grid ----------------
   int ix, jy, kz;
   if (MODE == 2) {
-    ix = tsc_base_index(x, P.xl, B.d[0], B.n[0]);
-    jy = tsc_base_index(y, P.xl, B.d[1], B.n[1]);
-    kz = tsc_base_index(z, P.xl, B.d[2], B.n[2]);
+    ix = tsc_base_index(x, P.xl, B.d[0], B.inv_d[0], B.n[0]);
+    jy = tsc_base_index(y, P.xl, B.d[1], B.inv_d[1], B.n[1]);
+    kz = tsc_base_index(z, P.xl, B.d[2], B.inv_d[2], B.n[2]);
   } else {
-    float tx = (x - P.xl) / B.d[0], ty = (y - P.xl) / B.d[1], tz = (z - P.xl) / B.d[2];
+    const float tx = index_quotient(x - P.xl, B.d[0], B.inv_d[0]);
+    const float ty = index_quotient(y - P.xl, B.d[1], B.inv_d[1]);
+    const float tz = index_quotient(z - P.xl, B.d[2], B.inv_d[2]);
     // INT() is defined only on a bounded range; beyond it the reference would
     // index outside U1 anyway
     if (!(tx > -1.0e9f && tx < 1.0e9f && ty > -1.0e9f && ty < 1.0e9f && tz > -1.0e9f && tz < 1.0e9f)) {
@@ -178,16 +191,16 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
       int lo[3], hi[3], ci[3];
       int irk = 0;
       if (MODE == 2) {
-        ci[0] = (int)((x - G.org[0]) / G.d[0]);
-        ci[1] = (int)((y - G.org[1]) / G.d[1]);
-        ci[2] = (int)((z - G.org[2]) / G.d[2]);
+        ci[0] = (int)index_quotient(x - G.org[0], G.d[0], G.inv_d[0]);
+        ci[1] = (int)index_quotient(y - G.org[1], G.d[1], G.inv_d[1]);
+        ci[2] = (int)index_quotient(z - G.org[2], G.d[2], G.inv_d[2]);
         member(g, ci[0], ci[1], ci[2]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) { lo[a] = max(ci[a] - 1, 1); hi[a] = min(ci[a] + 1, G.n[a]); }
       } else {
-        ci[0] = (int)floorf((x - G.org[0]) / G.d[0]) + 1;
-        ci[1] = (int)floorf((y - G.org[1]) / G.d[1]) + 1;
-        ci[2] = (int)floorf((z - G.org[2]) / G.d[2]) + 1;
+        ci[0] = (int)floorf(index_quotient(x - G.org[0], G.d[0], G.inv_d[0])) + 1;
+        ci[1] = (int)floorf(index_quotient(y - G.org[1], G.d[1], G.inv_d[1])) + 1;
+        ci[2] = (int)floorf(index_quotient(z - G.org[2], G.d[2], G.inv_d[2])) + 1;
         member(g, ci[0], ci[1], ci[2]);
         irk = max(G.level - species, 0);
         const int nk = 1 << irk;

@@ -459,18 +459,18 @@ __device__ __forceinline__ float glb_load(float* a) {
   return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// 27-cell deposit of one particle into the LDS tile.  lx/ly/lz: the tile-local
-// index of the three cells per axis, vx/vy/vz: 3-bit masks of which of them
-// lie in this tile (and in the grid), w*: the weights per axis.  The terms are
-// the reference's FP32 products ((m*wx)*wy)*wz.  Deterministic mode scales the
-// mass by the power-of-two fixed-point scale first (exact), converts every
-// term to the nearest integer and gives the centre cell the remainder Q(m) - (sum of
-// the other 26), so each particle deposits exactly Q(m) on a periodic grid.
-template <int RED, int TX, int TY, int TZ>
-__device__ __forceinline__ void deposit27(typename AccT<RED>::T* acc, float m, const int lx[3],
-                                          const int ly[3], const int lz[3], unsigned vx, unsigned vy,
-                                          unsigned vz, const float wx[3], const float wy[3],
-                                          const float wz[3], float scale) {
+// 27-cell deposit of one particle into the LDS tile, which carries a 2-cell
+// halo on every side: the particle's centre cell lies within one cell of the
+// tile (it was binned to the tile because one of its 27 cells is inside), so
+// all 27 cells are in the array and the deposit needs no per-cell test, one
+// base address and 27 constant offsets.  The terms are the reference's FP32
+// products ((m*wx)*wy)*wz.  Deterministic mode scales the mass by the
+// power-of-two fixed-point scale first (exact), rounds every term to an
+// integer and gives the centre cell the remainder Q(m) - (sum of the other
+// 26), so each particle deposits exactly Q(m) over its stencil.
+template <int RED, int HX, int HY>
+__device__ __forceinline__ void deposit27(typename AccT<RED>::T* base, float m, const float wx[3],
+                                          const float wy[3], const float wz[3], float scale) {
   const float ms = RED == 0 ? m * scale : m;
   float mx[3], mxy[3][3];
 #pragma unroll
@@ -486,8 +486,7 @@ __device__ __forceinline__ void deposit27(typename AccT<RED>::T* acc, float m, c
       for (int b = 0; b < 3; ++b)
 #pragma unroll
         for (int a = 0; a < 3; ++a)
-          if ((vx >> a) & (vy >> b) & (vz >> c) & 1u)
-            lds_add(reinterpret_cast<float*>(acc) + (lx[a] + TX * (ly[b] + TY * lz[c])), mxy[b][a] * wz[c]);
+          lds_add(reinterpret_cast<float*>(base) + (a + HX * (b + HY * c)), mxy[b][a] * wz[c]);
   } else {
     int rest = 0;
 #pragma unroll
@@ -499,11 +498,9 @@ __device__ __forceinline__ void deposit27(typename AccT<RED>::T* acc, float m, c
           if (a == 1 && b == 1 && c == 1) continue;
           const int q = __float2int_rn(mxy[b][a] * wz[c]);
           rest += q;
-          if ((vx >> a) & (vy >> b) & (vz >> c) & 1u)
-            lds_add(reinterpret_cast<long long*>(acc) + (lx[a] + TX * (ly[b] + TY * lz[c])), (long long)q);
+          lds_add(reinterpret_cast<long long*>(base) + (a + HX * (b + HY * c)), (long long)q);
         }
-    if ((vx >> 1) & (vy >> 1) & (vz >> 1) & 1u)
-      lds_add(reinterpret_cast<long long*>(acc) + (lx[1] + TX * (ly[1] + TY * lz[1])), (long long)(__float2int_rn(ms) - rest));
+    lds_add(reinterpret_cast<long long*>(base) + (1 + HX * (1 + HY * 1)), (long long)(__float2int_rn(ms) - rest));
   }
 }
 
@@ -519,11 +516,22 @@ __device__ __forceinline__ float base_centre(const Plan& P, int a, int i, int n)
   return r[i];
 }
 
+// tile-local coordinate of the stencil centre: on the periodic base grid the
+// image of the (wrapped) centre that lies next to the tile
+__device__ __forceinline__ int centre_local(int c, int org, int ext, int n, bool periodic) {
+  if (!periodic) return c - org;
+  int l = wrap1(c, n) - org;
+  if (l > ext) l -= n;
+  if (l < -1) l += n;
+  return l;
+}
+
 template <int MODE, int RED, int TX, int TY, int TZ>
 __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
   using Acc = typename AccT<RED>::T;
   constexpr int T = TX * TY * TZ;
-  __shared__ Acc acc[T];
+  constexpr int HX = TX + 4, HY = TY + 4, HZ = TZ + 4, H = HX * HY * HZ;
+  __shared__ Acc acc[H];                 // the tile with a 2-cell halo
   __shared__ unsigned s_item;
   __shared__ int s_last;
   __shared__ long long s_redi[16];
@@ -531,6 +539,8 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const float scale = (float)D.scale;
   const float k0[3] = {P.ktab[P.kmax - 1], P.ktab[P.kmax], P.ktab[P.kmax + 1]};
+  // interior cell (a, b, c) of the tile
+  auto at = [](int a, int b, int c) { return (a + 2) + HX * ((b + 2) + HY * (c + 2)); };
 
   // workgroup w serves queue w mod n_queues, which hands out items
   // q, q + n_queues, ...; consecutive workgroups land on different XCDs, so
@@ -549,16 +559,16 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
     const int tx = local % G.nt[0], ty = (local / G.nt[0]) % G.nt[1], tz = local / (G.nt[0] * G.nt[1]);
     const int i0 = tx * TX + 1, j0 = ty * TY + 1, k0c = tz * TZ + 1;
     const int ex = min(TX, G.n[0] - i0 + 1), ey = min(TY, G.n[1] - j0 + 1), ez = min(TZ, G.n[2] - k0c + 1);
+    const bool periodic = g == 0;
 
-    for (int c = tid; c < T; c += nthr) acc[c] = 0;
+    for (int c = tid; c < H; c += nthr) acc[c] = 0;
     __syncthreads();
 
     if (!(it.flags & 1)) {
-      // ---- narrow entries: one lane per entry, 27 cells; two entries per
-      // lane per pass with both loads in flight before the first is used ----
-      // each lane walks its own contiguous stretch of the item, so the 64
-      // lanes of a wave hold particles from across the tile's list rather
-      // than 64 neighbours (fewer LDS address and bank conflicts)
+      // ---- narrow entries: one lane per entry, 27 cells; each lane walks
+      // its own contiguous stretch of the item (the 64 lanes of a wave hold
+      // particles from across the tile's list), two entries per pass with
+      // both loads in flight before the first is used ----
       const long long len = it.end - it.begin;
       const long long per = (len + nthr - 1) / nthr;
       const long long lane_end = min(it.begin + (long long)(tid + 1) * per, it.end);
@@ -568,55 +578,43 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
         const int4 v1 = e1 < lane_end ? D.entries[e1] : make_int4(0, 0, 0, 0);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-        if (u == 1 && e1 >= lane_end) break;
-        const int4 v = u == 0 ? v0 : v1;
-        int ci[3];
-        float wx[3], wy[3], wz[3], m;
-        if (MODE == 2) {
-          const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
-          m = __int_as_float(v.w);
-          float* w[3] = {wx, wy, wz};
-          if (g == 0) {
+          if (u == 1 && e1 >= lane_end) break;
+          const int4 v = u == 0 ? v0 : v1;
+          int ci[3];
+          float wx[3], wy[3], wz[3], m;
+          if (MODE == 2) {
+            const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
+            m = __int_as_float(v.w);
+            float* w[3] = {wx, wy, wz};
+            if (g == 0) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.n[a]);
-              tsc_weights_r(p[a], base_centre(P, a, ci[a] - 1, G.n[a]), base_centre(P, a, ci[a], G.n[a]),
-                            base_centre(P, a, ci[a] + 1, G.n[a]), G.inv_d[a], w[a]);
+              for (int a = 0; a < 3; ++a) {
+                ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]);
+                tsc_weights_r(p[a], base_centre(P, a, ci[a] - 1, G.n[a]), base_centre(P, a, ci[a], G.n[a]),
+                              base_centre(P, a, ci[a] + 1, G.n[a]), G.inv_d[a], w[a]);
+              }
+            } else {
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                ci[a] = (int)index_quotient(p[a] - G.org[a], G.d[a], G.inv_d[a]);
+                tsc_weights_r(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
+                              minirad(G.pr[a], ci[a] + 1, G.d[a]), G.inv_d[a], w[a]);
+              }
             }
           } else {
+            int irk;
+            unpack_table(v, ci[0], ci[1], ci[2], irk, m);
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              ci[a] = (int)((p[a] - G.org[a]) / G.d[a]);
-              tsc_weights_r(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
-                            minirad(G.pr[a], ci[a] + 1, G.d[a]), G.inv_d[a], w[a]);
-            }
+            for (int k = 0; k < 3; ++k) { wx[k] = k0[k]; wy[k] = k0[k]; wz[k] = k0[k]; }
           }
-        } else {
-          int irk;
-          unpack_table(v, ci[0], ci[1], ci[2], irk, m);
-#pragma unroll
-          for (int k = 0; k < 3; ++k) { wx[k] = k0[k]; wy[k] = k0[k]; wz[k] = k0[k]; }
-        }
-        int lx[3], ly[3], lz[3];
-        unsigned vmask[3] = {0u, 0u, 0u};
-        const int org[3] = {i0, j0, k0c};
-        int* l[3] = {lx, ly, lz};
-        const int ext[3] = {ex, ey, ez};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-#pragma unroll
-          for (int k = 0; k < 3; ++k) {
-            int c = ci[a] - 1 + k;
-            bool ok = true;
-            if (g == 0) c = wrap1(c, G.n[a]);
-            else ok = c >= 1 && c <= G.n[a];
-            const int loc = c - org[a];
-            ok = ok && loc >= 0 && loc < ext[a];
-            l[a][k] = loc;
-            vmask[a] |= (ok ? 1u : 0u) << k;
-          }
-        }
-        deposit27<RED, TX, TY, TZ>(acc, m, lx, ly, lz, vmask[0], vmask[1], vmask[2], wx, wy, wz, scale);
+          // in [-1, ext] by construction of the binning; clamped so the array
+          // bounds never depend on that
+          const int lx = clampi(centre_local(ci[0], i0, ex, G.n[0], periodic), -1, TX);
+          const int ly = clampi(centre_local(ci[1], j0, ey, G.n[1], periodic), -1, TY);
+          const int lz = clampi(centre_local(ci[2], k0c, ez, G.n[2], periodic), -1, TZ);
+          // corner (-1,-1,-1) of the stencil; halo coordinates are local + 2
+          Acc* base = acc + ((lx + 1) + HX * ((ly + 1) + HY * (lz + 1)));
+          deposit27<RED, HX, HY>(base, m, wx, wy, wz, scale);
         }
       }
     } else if (MODE == 1) {
@@ -633,12 +631,12 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
         const int za = max(max(ci[2] - nk, 1), k0c), zb = min(min(ci[2] + nk, G.n[2]), k0c + ez - 1);
         if (xb < xa || yb < ya || zb < za) continue;
         const int bx = xb - xa + 1, by = yb - ya + 1, total = bx * by * (zb - za + 1);
+        const float ms = RED == 0 ? m * scale : m;
         for (int c = lane; c < total; c += 64) {
           const int a = c % bx, r = c / bx, b = r % by, cc = r / by;
           const int ii = xa + a, jj = ya + b, kk = za + cc;
-          const float ms = RED == 0 ? m * scale : m;
           const float term = ((ms * K[ii - ci[0]]) * K[jj - ci[1]]) * K[kk - ci[2]];
-          Acc* dst = acc + ((ii - i0) + TX * ((jj - j0) + TY * (kk - k0c)));
+          Acc* dst = acc + at(ii - i0, jj - j0, kk - k0c);
           if (RED == 0) lds_add(reinterpret_cast<long long*>(dst), (long long)__float2int_rn(term));
           else lds_add(reinterpret_cast<float*>(dst), term);
         }
@@ -646,15 +644,30 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
     }
     __syncthreads();
 
-    // ---- split tile: every item stores its partial tile; the last one to
-    // finish sums the partials in item order (fixed order: deterministic in
-    // both reductions) and writes the cells ----
+    // ---- a tile spanning a whole periodic axis: its halo cells are its own
+    // interior cells under the reference's wrap; fold them in ----
+    if (periodic && (G.nt[0] == 1 || G.nt[1] == 1 || G.nt[2] == 1)) {
+      for (int c = tid; c < H; c += nthr) {
+        const int a = c % HX - 2, b = (c / HX) % HY - 2, cz = c / (HX * HY) - 2;
+        if (a >= 0 && a < ex && b >= 0 && b < ey && cz >= 0 && cz < ez) continue;
+        const int ga = wrap1(i0 + a, G.n[0]) - i0, gb = wrap1(j0 + b, G.n[1]) - j0,
+                  gc = wrap1(k0c + cz, G.n[2]) - k0c;
+        if (ga < 0 || ga >= ex || gb < 0 || gb >= ey || gc < 0 || gc >= ez) continue;
+        const Acc v = acc[c];
+        if (v != 0) lds_add(acc + at(ga, gb, gc), v);
+      }
+      __syncthreads();
+    }
+
+    // ---- split tile: every item stores its partial interior; the last one
+    // to finish sums the partials in item order (fixed order: deterministic
+    // in both reductions) and writes the cells ----
     const int nit = D.nitems_tile[t];
     bool writer = true;
     if (nit > 1) {
       Acc* part = reinterpret_cast<Acc*>(D.scratch) + (long long)D.split_slot[t] * T;
       Acc* mine = part + (long long)(it.flags >> 8) * T;
-      for (int c = tid; c < T; c += nthr) mine[c] = acc[c];
+      for (int c = tid; c < T; c += nthr) mine[c] = acc[at(c % TX, (c / TX) % TY, c / (TX * TY))];
       __threadfence();
       __syncthreads();
       if (tid == 0) {
@@ -668,7 +681,7 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
         for (int c = tid; c < T; c += nthr) {
           Acc v = 0;
           for (int k = 0; k < nit; ++k) v += glb_load(part + (long long)k * T + c);
-          acc[c] = v;
+          acc[at(c % TX, (c / TX) % TY, c / (TX * TY))] = v;
         }
         __syncthreads();
       }
@@ -681,7 +694,7 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
       for (int c = tid; c < T; c += nthr) {
         const int a = c % TX, b = (c / TX) % TY, cz = c / (TX * TY);
         if (a >= ex || b >= ey || cz >= ez) continue;
-        const Acc v = acc[c];
+        const Acc v = acc[at(a, b, cz)];
         if (RED == 0) msi += (long long)v;
         else msd += (double)v;
         float f = RED == 0 ? (float)((double)v * inv_scale) : (float)v;
@@ -741,7 +754,7 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
       float w[3][3], dw[3][3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.n[a]);
+        ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]);
         const float c0 = rad[a][ci[a] - 1], c1 = rad[a][ci[a]], c2 = rad[a][ci[a] + 1];
         tsc_weights(p[a], c0, c1, c2, G.d[a], w[a]);
         tsc_dweights(p[a], c0, c1, c2, G.d[a], dw[a]);
@@ -777,7 +790,7 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
         float w[3][3], dw[3][3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          ci[a] = (int)((p[a] - G.org[a]) / G.d[a]);
+          ci[a] = (int)index_quotient(p[a] - G.org[a], G.d[a], G.inv_d[a]);
           const float c0 = minirad(G.pr[a], ci[a] - 1, G.d[a]), c1 = minirad(G.pr[a], ci[a], G.d[a]),
                       c2 = minirad(G.pr[a], ci[a] + 1, G.d[a]);
           tsc_weights(p[a], c0, c1, c2, G.d[a], w[a]);
@@ -841,7 +854,7 @@ __global__ void __launch_bounds__(256) plain_base(Plan P, Parts A, float* u1) {
       const float* rad[3] = {P.radx, P.rady, P.radz};
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.n[a]);
+        ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]);
         tsc_weights(p[a], rad[a][ci[a] - 1], rad[a][ci[a]], rad[a][ci[a] + 1], G.d[a], w[a]);
       }
     } else {
