@@ -63,7 +63,9 @@ class SnapshotSpec:
         return len(self.patches_per_level)
 
 
-# The workloads of the density build (see DESIGN.md for the mapping).
+# The workloads of the density build: 1 is the small CPU-runnable case, 2 the
+# 256^3-particle single-GPU benchmark, 3 and 5 the large multi-level runs, 4 the
+# base grid alone (no refinement).
 CONFIGS = {
     1: SnapshotSpec("c1_128cubed_3lev", 128, 40.0, 128, (24, 18, 12), 200, 0.70, 1e11, 1e14, 1),
     2: SnapshotSpec("c2_256cubed_4lev", 256, 40.0, 256, (320, 260, 200, 140), 200, 0.70,
@@ -359,3 +361,94 @@ def make_snapshot(spec: SnapshotSpec, n_side: int | None = None, nx: int | None 
     return Snapshot(np.ascontiguousarray(x[order]), np.ascontiguousarray(y[order]),
                     np.ascontiguousarray(z[order]), np.ascontiguousarray(m[order]),
                     npart_esp, geom, spec)
+
+
+def make_snapshot_device(spec: SnapshotSpec, device="cuda:0", chunk: int = 1 << 26,
+                         interp_degree: int = 2) -> Snapshot:
+    """The same workload generated on the GPU (for the 134M- and 1.07G-particle
+    configurations, which numpy cannot produce in reasonable time or memory).
+    The halo catalogue and the AMR hierarchy come from the same host RNG stream
+    as make_snapshot; the particles from torch's device RNG seeded with the
+    workload seed, so they differ from make_snapshot's for the same spec.
+    Returns a Snapshot whose x, y, z, m are device tensors in reference order
+    (species contiguous) and whose npart_esp is a host array."""
+    import torch
+
+    rng = np.random.default_rng(spec.seed)
+    n = spec.n
+    nh, conc, centre, rvir = _halo_catalogue(spec, rng, n)
+    rote, rete = background(spec.h, spec.omega_m, 0.0)
+    lado0 = f32(spec.box)
+    g0 = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, np.zeros(1, np.int32),
+                  np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0))
+    npatch, pnx, pny, pnz, prx, pry, prz, pare = _tuned_hierarchy(
+        spec, {"nx": spec.nx, "radx": g0.radx, "dx": g0.dx}, (nh, conc, centre, rvir), spec.seed, n)
+    geom = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, npatch, pnx, pny, pnz,
+                    prx, pry, prz, pare=pare, interp_degree=interp_degree)
+
+    dev = torch.device(device)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(spec.seed)
+    L = float(spec.box)
+    pos = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(3)]
+    n_in = int(nh.sum())
+    conc_t = torch.from_numpy(conc).to(dev, torch.float32)
+    rs_t = torch.from_numpy(rvir / conc).to(dev, torch.float32)
+    cen_t = torch.from_numpy(centre).to(dev, torch.float32)
+    starts = np.concatenate([[0], np.cumsum(nh)])
+    for s0 in range(0, n_in, chunk):
+        s1 = min(s0 + chunk, n_in)
+        idx = torch.arange(s0, s1, device=dev)
+        hid = torch.searchsorted(torch.from_numpy(starts[1:]).to(dev), idx, right=True)
+        c = conc_t[hid]
+        target = torch.rand(s1 - s0, generator=gen, device=dev) * (torch.log1p(c) - c / (1 + c))
+        lo, hi = torch.zeros_like(c), c.clone()
+        for _ in range(32):                       # NFW radii by bisection
+            mid = 0.5 * (lo + hi)
+            below = (torch.log1p(mid) - mid / (1 + mid)) < target
+            lo = torch.where(below, mid, lo)
+            hi = torch.where(below, hi, mid)
+        r = 0.5 * (lo + hi) * rs_t[hid]
+        cost = torch.rand(s1 - s0, generator=gen, device=dev) * 2 - 1
+        phi = torch.rand(s1 - s0, generator=gen, device=dev) * (2 * np.pi)
+        sint = torch.sqrt(torch.clamp(1 - cost * cost, min=0))
+        comp = (sint * torch.cos(phi), sint * torch.sin(phi), cost)
+        for a in range(3):
+            v = cen_t[hid, a] + r * comp[a]
+            pos[a][s0:s1] = torch.remainder(v + L / 2, L) - L / 2
+        del idx, hid, c, target, lo, hi, r, cost, phi, sint, comp
+    for s0 in range(n_in, n, chunk):
+        s1 = min(s0 + chunk, n)
+        for a in range(3):
+            pos[a][s0:s1] = torch.rand(s1 - s0, generator=gen, device=dev) * L - L / 2
+    half = float(f32(spec.box / 2))
+    for a in range(3):
+        pos[a][pos[a] >= half] = -half
+    mp = f32(f32(rote * f32(rete * f32(rete * rete))) * f32(f32(spec.box) ** 3) / f32(n))
+    # species by the local-density rule on an NX-sized NGP grid (equal masses:
+    # the density is count * m / (DX**3 * RODO * RE0**3))
+    nx = spec.nx
+    dx = float(geom.dx)
+    xmin = float(f32(-f32(nx) * geom.dx / f32(2.0)))
+    flat = torch.zeros(n, dtype=torch.int64, device=dev)
+    for a in (2, 1, 0):
+        i = torch.clamp(((pos[a] - xmin) / dx).to(torch.int64), 0, nx - 1)
+        flat = flat * nx + i
+        del i
+    cnt = torch.bincount(flat, minlength=nx ** 3)
+    bas = float(f32(f32(f32(geom.dx * geom.dx) * geom.dx) * rote * f32(rete * f32(rete * rete))))
+    dens = cnt[flat].to(torch.float32) * float(mp) / bas
+    del cnt, flat
+    s = torch.where(dens > 0, torch.log(torch.clamp(dens, min=1e-30)) / np.log(8.0),
+                    torch.zeros_like(dens))
+    s = torch.clamp(torch.trunc(s), 0, N_ESP - 1).to(torch.int32)
+    del dens
+    npart_esp = torch.bincount(s, minlength=N_ESP).cpu().numpy().astype(np.int32)
+    order = torch.sort(s, stable=True).indices
+    del s
+    out = [p[order].contiguous() for p in pos]
+    del pos, order
+    m = torch.full((n,), float(mp), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize(dev)
+    torch.cuda.empty_cache()
+    return Snapshot(out[0], out[1], out[2], m, npart_esp, geom, spec)

@@ -57,7 +57,7 @@ def main() -> int:
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the density build of one snapshot runs on a single "
-              "MI355X (how it would shard across GPUs is described in DESIGN.md); "
+              "MI355X (multi-GPU sharding of the build is not implemented); "
               "run it with --gpus 1", file=sys.stderr)
         return 2
 
