@@ -132,6 +132,34 @@ struct HostPlan {
   long long u1_cells = 0;
 };
 
+// Every value the kernels of one batched call receive.  Zero-initialised
+// before it is filled, so two calls with equal arguments compare equal byte
+// for byte: a captured graph is replayed only for exactly these values.
+struct PipeArgs {
+  Plan P;
+  Parts A;
+  BlockTables BT;
+  DepArgs D;
+  Counters* C;
+  unsigned* cnt;
+  unsigned long long* start;
+  unsigned long long* cursor;
+  unsigned long long* bsum;
+  unsigned* nitems;
+  unsigned* nsplit;
+  unsigned long long* item_off;
+  unsigned long long* split_off;
+  int* split;
+  unsigned* done;
+  double* grid_mass;
+  long long nblk;
+  long long nkeys;
+  unsigned long long cap_entries;   // entries, items and split-tile slots the buffers hold
+  unsigned long long cap_items;
+  unsigned long long cap_split;
+  int ntiles, chunk, chunk_wide, red, grid, block, cus, elem;
+};
+
 struct asohf_ctx {
   int device = 0;
   int cus = 0;
@@ -143,10 +171,14 @@ struct asohf_ctx {
   DBuf cnt, start, cursor, bsum, counters, done, nitems, split, tmass, entries, items, scratch;
   DBuf flags, idx, pool, pool_off, pool_nk, blk_pairs, blk_mmax, nsplit, item_off, split_off;
   long long pool_want = 0;   // pairs the last count pass needed
+  bool sized = false;        // the entry/item/scratch buffers were sized by an exact call
+  hipGraphExec_t gexec = nullptr;   // the captured batched build and the arguments it holds
+  PipeArgs gkey;
   DBuf h_in[4], h_out[2], h_mass;    // device copies for the host entry point
   Counters* hcount = nullptr;          // pinned
   unsigned long long* hscalar = nullptr;
-  hipEvent_t ev[8] = {};   // 0 start, 1 count pass, 2 scatter start, 3 scatter end, 4/5 deposit, 6 end
+  hipEvent_t ev[8] = {};   // 0 start, 1 count pass, 2 scatter start, 3 scatter end, 4/5 deposit, 6 end,
+                           // 7 the caller's stream before a graph launch
   asohf_stats stats{};
   bool stats_pending = false;
   bool stats_phases = false;
@@ -555,16 +587,23 @@ static asohf_status decode_err(asohf_ctx* ctx, unsigned e) {
 // ---------------------------------------------------------------------------
 // batched pipeline
 // ---------------------------------------------------------------------------
-static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, float* u1, float* u11,
-                                double* grid_mass, hipStream_t s) {
+// Workspaces whose size depends only on the geometry and the particle count,
+// and the arguments of every kernel of the call.  The entry, item and scratch
+// buffers keep the size of the largest earlier call; the gate kernel checks
+// them on the device.
+static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, float* u1, float* u11,
+                                 double* grid_mass, PipeArgs& a) {
   HostPlan& H = ctx->plan;
-  const Plan& P = H.dev;
-  const int ntiles = P.ntiles;
-  const long long nkeys = 2ll * ntiles;
-  const int T = P.tile[0] * P.tile[1] * P.tile[2];
-  const int red = o.reduction;
-  if (A.n > (1ll << 32)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^32 particles");
-  const long long nblk = bin_blocks(A.n);
+  memset(&a, 0, sizeof(a));
+  a.P = H.dev;
+  a.A = A;
+  a.ntiles = a.P.ntiles;
+  a.nkeys = 2ll * a.ntiles;
+  a.red = o.reduction;
+  a.elem = a.red == 0 ? 8 : 4;
+  a.nblk = bin_blocks(A.n);
+  const long long nkeys = a.nkeys, nblk = a.nblk;
+  const int ntiles = a.ntiles;
   HIPCHK(ctx->cnt.ensure(sizeof(unsigned) * std::max<long long>(nkeys, 1)), "allocating counts");
   HIPCHK(ctx->start.ensure(sizeof(unsigned long long) * (nkeys + 1)), "allocating offsets");
   HIPCHK(ctx->cursor.ensure(sizeof(unsigned long long) * std::max<long long>(nkeys, 1)), "allocating cursors");
@@ -585,106 +624,210 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
   HIPCHK(ctx->pool_nk.ensure(sizeof(int) * std::max<long long>(nblk, 1)), "allocating block tables");
   HIPCHK(ctx->blk_pairs.ensure(sizeof(unsigned long long) * std::max<long long>(nblk, 1)), "allocating block sums");
   HIPCHK(ctx->blk_mmax.ensure(sizeof(unsigned) * std::max<long long>(nblk, 1)), "allocating block sums");
-  BlockTables BT;
-  BT.pool = ctx->pool.as<int2>();
-  BT.cap = (long long)(ctx->pool.cap / sizeof(int2));
-  BT.off = ctx->pool_off.as<long long>();
-  BT.nkeys = ctx->pool_nk.as<int>();
-  BT.pairs = ctx->blk_pairs.as<unsigned long long>();
-  BT.mmax = ctx->blk_mmax.as<unsigned>();
-  Counters* C = ctx->counters.as<Counters>();
-  const int chunk = std::max(ctx->tune.chunk, 64);
-  const int chunk_wide = std::max(chunk / 64, 4);
-  unsigned long long* start = ctx->start.as<unsigned long long>();
-  unsigned long long* bsum = ctx->bsum.as<unsigned long long>();
-
-  HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
-  HIPCHK(launch_fill_u32((unsigned*)C, 0u, sizeof(Counters) / 4, s), "clearing counters");
-  HIPCHK(launch_fill_u32(ctx->cnt.as<unsigned>(), 0u, nkeys, s), "clearing counts");
-  HIPCHK(launch_fill_u32(ctx->done.as<unsigned>(), 0u, ntiles, s), "clearing tile state");
-  HIPCHK(launch_count(P, A, ctx->cnt.as<unsigned>(), C, BT, ctx->cus, s), "count kernel");
-  if (nblk > 0) HIPCHK(launch_reduce_blocks(BT, nblk, C, s), "block sums");
-  HIPCHK(launch_scan(ctx->cnt.as<unsigned>(), nkeys, bsum, start, ctx->cursor.as<unsigned long long>(), s),
-         "scan");
-  HIPCHK(launch_plan_count(start, ntiles, chunk, chunk_wide, ctx->nitems.as<unsigned>(),
-                           ctx->nsplit.as<unsigned>(), s), "planning items");
-  HIPCHK(launch_scan(ctx->nitems.as<unsigned>(), ntiles, bsum, ctx->item_off.as<unsigned long long>(), nullptr, s),
-         "item offsets");
-  HIPCHK(launch_scan(ctx->nsplit.as<unsigned>(), ntiles, bsum, ctx->split_off.as<unsigned long long>(), nullptr, s),
-         "split offsets");
-  HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
-  HIPCHK(hipMemcpyAsync(ctx->hcount, C, sizeof(Counters), hipMemcpyDeviceToHost, s), "reading counters");
-  HIPCHK(hipMemcpyAsync(ctx->hscalar, start + nkeys, sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
-         "reading totals");
-  HIPCHK(hipMemcpyAsync(ctx->hscalar + 1, ctx->item_off.as<unsigned long long>() + ntiles,
-                        sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "reading totals");
-  HIPCHK(hipMemcpyAsync(ctx->hscalar + 2, ctx->split_off.as<unsigned long long>() + ntiles,
-                        sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "reading totals");
-  HIPCHK(hipStreamSynchronize(s), "count pass");
-  const Counters& hc = *ctx->hcount;
-  const unsigned long long n_entries = ctx->hscalar[0];
-  const unsigned long long n_items = ctx->hscalar[1];
-  const unsigned long long n_split = ctx->hscalar[2];
-  asohf_status st = decode_err(ctx, hc.err);
-  if (st != ASOHF_OK) return st;
-  if (n_items >= (1ull << 31)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^31 work items");
-  // fixed-point scale: the power of two with max(m) * scale in [2^30, 2^31),
-  // so every quantised term fits an int32 and any sum over <= 2^32 particles
-  // an int64
-  const float mmax = __builtin_bit_cast(float, hc.mmax_bits);
-  int e2 = 0;
-  if (mmax > 0.f) (void)std::frexp(mmax, &e2);
-  const double scale = std::ldexp(1.0, 31 - e2);
-  unsigned long long sub_max = 0;
-  for (int k = 0; k < kPoolSubs; ++k) sub_max = std::max(sub_max, hc.pool_fill[k * (kLine / 2)]);
-  ctx->pool_want = (long long)(sub_max * kPoolSubs);
-  HIPCHK(ctx->entries.ensure(sizeof(int4) * std::max<unsigned long long>(n_entries, 1)), "allocating entries");
-  HIPCHK(ctx->items.ensure(sizeof(Item) * std::max<unsigned long long>(n_items, 1)), "allocating work items");
-  HIPCHK(ctx->scratch.ensure((size_t)std::max<unsigned long long>(n_split, 1) * T * (red == 0 ? 8 : 4)),
-         "allocating split-tile scratch");
-  HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
-  HIPCHK(launch_scatter(P, A, ctx->cursor.as<unsigned long long>(), ctx->entries.as<int4>(), n_entries, BT,
-                        ctx->cus, s),
-         "scatter kernel");
-  HIPCHK(hipEventRecord(ctx->ev[3], s), "event");
-  HIPCHK(launch_plan_write(start, ntiles, chunk, chunk_wide, ctx->item_off.as<unsigned long long>(),
-                           ctx->split_off.as<unsigned long long>(), ctx->items.as<Item>(), ctx->split.as<int>(), s),
-         "writing items");
-  DepArgs D;
-  memset(&D, 0, sizeof(D));
-  D.entries = ctx->entries.as<int4>();
-  D.items = ctx->items.as<Item>();
-  D.nitems_tile = ctx->nitems.as<int>();
-  D.split_slot = ctx->split.as<int>();
-  D.done = ctx->done.as<unsigned>();
-  D.scratch = ctx->scratch.p;
-  D.C = C;
-  D.u1 = u1;
-  D.u11 = u11;
-  D.tile_mass_i = ctx->tmass.as<long long>();
-  D.tile_mass_d = ctx->tmass.as<double>();
-  D.scale = scale;
-  D.inv_scale = 1.0 / scale;
-  D.n_items = (unsigned)n_items;
+  a.BT.pool = ctx->pool.as<int2>();
+  a.BT.cap = (long long)(ctx->pool.cap / sizeof(int2));
+  a.BT.off = ctx->pool_off.as<long long>();
+  a.BT.nkeys = ctx->pool_nk.as<int>();
+  a.BT.pairs = ctx->blk_pairs.as<unsigned long long>();
+  a.BT.mmax = ctx->blk_mmax.as<unsigned>();
+  a.C = ctx->counters.as<Counters>();
+  a.cnt = ctx->cnt.as<unsigned>();
+  a.start = ctx->start.as<unsigned long long>();
+  a.cursor = ctx->cursor.as<unsigned long long>();
+  a.bsum = ctx->bsum.as<unsigned long long>();
+  a.nitems = ctx->nitems.as<unsigned>();
+  a.nsplit = ctx->nsplit.as<unsigned>();
+  a.item_off = ctx->item_off.as<unsigned long long>();
+  a.split_off = ctx->split_off.as<unsigned long long>();
+  a.split = ctx->split.as<int>();
+  a.done = ctx->done.as<unsigned>();
+  a.grid_mass = grid_mass;
+  a.chunk = std::max(ctx->tune.chunk, 64);
+  a.chunk_wide = std::max(a.chunk / 64, 4);
+  a.cus = ctx->cus;
   const int waves = std::max(ctx->tune.waves_per_cu, 1);
-  const int block = ctx->tune.block;
-  int grid = std::max(1, ctx->cus * waves / std::max(block / 64, 1));
-  grid = (int)std::min<long long>(grid, std::max<unsigned long long>(n_items, 1));
-  D.n_queues = std::min(kQueues, grid);
-  HIPCHK(hipEventRecord(ctx->ev[4], s), "event");
-  HIPCHK(launch_deposit(P, D, red, grid, block, s), "deposit kernel");
-  HIPCHK(hipEventRecord(ctx->ev[5], s), "event");
-  if (grid_mass) HIPCHK(launch_grid_sums(P, D, red, grid_mass, s), "grid sums");
-  HIPCHK(hipEventRecord(ctx->ev[6], s), "event");
+  a.block = ctx->tune.block;
+  a.grid = std::max(1, ctx->cus * waves / std::max(a.block / 64, 1));
+  a.D.nitems_tile = ctx->nitems.as<int>();
+  a.D.split_slot = ctx->split.as<int>();
+  a.D.done = ctx->done.as<unsigned>();
+  a.D.C = a.C;
+  a.D.u1 = u1;
+  a.D.u11 = u11;
+  a.D.tile_mass_i = ctx->tmass.as<long long>();
+  a.D.tile_mass_d = ctx->tmass.as<double>();
+  a.D.n_queues = std::min(kQueues, a.grid);
+  return ASOHF_OK;
+}
+
+// the buffers sized by the planning totals, as they are now
+static void bind_sized(asohf_ctx* ctx, PipeArgs& a) {
+  const int T = a.P.tile[0] * a.P.tile[1] * a.P.tile[2];
+  a.D.entries = ctx->entries.as<int4>();
+  a.D.items = ctx->items.as<Item>();
+  a.D.scratch = ctx->scratch.p;
+  a.cap_entries = ctx->entries.cap / sizeof(int4);
+  a.cap_items = ctx->items.cap / sizeof(Item);
+  a.cap_split = ctx->scratch.cap / ((size_t)T * a.elem);
+}
+
+// count pass: per-(tile, kind) counts, block sums, offsets and the item plan
+static asohf_status enqueue_count(asohf_ctx* ctx, const PipeArgs& a, hipStream_t s) {
+  HIPCHK(launch_fill_u32((unsigned*)a.C, 0u, sizeof(Counters) / 4, s), "clearing counters");
+  HIPCHK(launch_fill_u32(a.cnt, 0u, a.nkeys, s), "clearing counts");
+  HIPCHK(launch_fill_u32(a.done, 0u, a.ntiles, s), "clearing tile state");
+  HIPCHK(launch_count(a.P, a.A, a.cnt, a.C, a.BT, a.cus, s), "count kernel");
+  if (a.nblk > 0) HIPCHK(launch_reduce_blocks(a.BT, a.nblk, a.C, s), "block sums");
+  HIPCHK(launch_scan(a.cnt, a.nkeys, a.bsum, a.start, a.cursor, s), "scan");
+  HIPCHK(launch_plan_count(a.start, a.ntiles, a.chunk, a.chunk_wide, a.nitems, a.nsplit, s), "planning items");
+  HIPCHK(launch_scan(a.nitems, a.ntiles, a.bsum, a.item_off, nullptr, s), "item offsets");
+  HIPCHK(launch_scan(a.nsplit, a.ntiles, a.bsum, a.split_off, nullptr, s), "split offsets");
+  return ASOHF_OK;
+}
+
+// gate, scatter, item plan, persistent deposit, grid sums; events between the
+// phases unless the sequence is being captured
+static asohf_status enqueue_deposit(asohf_ctx* ctx, const PipeArgs& a, hipStream_t s, bool events) {
+  HIPCHK(launch_gate(a.start + a.nkeys, a.item_off + a.ntiles, a.split_off + a.ntiles, a.cap_entries,
+                     a.cap_items, a.cap_split, a.C, s), "gate");
+  if (events) HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
+  HIPCHK(launch_scatter(a.P, a.A, a.cursor, const_cast<int4*>(a.D.entries), a.cap_entries, a.BT, a.C, a.cus, s), "scatter kernel");
+  if (events) HIPCHK(hipEventRecord(ctx->ev[3], s), "event");
+  HIPCHK(launch_plan_write(a.start, a.ntiles, a.chunk, a.chunk_wide, a.item_off, a.split_off,
+                           const_cast<Item*>(a.D.items), a.split, a.C, s), "writing items");
+  if (events) HIPCHK(hipEventRecord(ctx->ev[4], s), "event");
+  HIPCHK(launch_deposit(a.P, a.D, a.red, a.grid, a.block, s), "deposit kernel");
+  if (events) HIPCHK(hipEventRecord(ctx->ev[5], s), "event");
+  if (a.grid_mass) HIPCHK(launch_grid_sums(a.P, a.D, a.red, a.grid_mass, s), "grid sums");
+  return ASOHF_OK;
+}
+
+static void set_stats(asohf_ctx* ctx, const Parts& A, const Counters& hc, bool phases) {
   ctx->stats = asohf_stats{};
   ctx->stats.n_particles = A.n;
   ctx->stats.n_pairs = (int64_t)hc.n_pairs;
-  ctx->stats.n_entries = (int64_t)n_entries;
-  ctx->stats.n_tiles = ntiles;
-  ctx->stats.n_items = (int64_t)n_items;
-  ctx->stats.n_cells = H.ncells;
+  ctx->stats.n_entries = (int64_t)hc.n_entries;
+  ctx->stats.n_tiles = ctx->plan.dev.ntiles;
+  ctx->stats.n_items = (int64_t)hc.n_items;
+  ctx->stats.n_cells = ctx->plan.ncells;
   ctx->stats_pending = true;
-  ctx->stats_phases = true;
+  ctx->stats_phases = phases;
+}
+
+static void note_pool(asohf_ctx* ctx, const Counters& hc) {
+  unsigned long long sub_max = 0;
+  for (int k = 0; k < kPoolSubs; ++k) sub_max = std::max(sub_max, hc.pool_fill[k * (kLine / 2)]);
+  ctx->pool_want = (long long)(sub_max * kPoolSubs);
+}
+
+// Exact form: the count pass, one host read of its totals, buffers grown to
+// fit, then the rest.  Used for the first call on new sizes and whenever the
+// buffers of an earlier call turn out too small.
+static asohf_status run_exact(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, float* u1, float* u11,
+                              double* grid_mass, hipStream_t s) {
+  PipeArgs a;
+  asohf_status st = prepare_args(ctx, o, A, u1, u11, grid_mass, a);
+  if (st != ASOHF_OK) return st;
+  HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
+  st = enqueue_count(ctx, a, s);
+  if (st != ASOHF_OK) return st;
+  HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
+  HIPCHK(hipMemcpyAsync(ctx->hcount, a.C, sizeof(Counters), hipMemcpyDeviceToHost, s), "reading counters");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar, a.start + a.nkeys, sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
+         "reading totals");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar + 1, a.item_off + a.ntiles, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s), "reading totals");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar + 2, a.split_off + a.ntiles, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s), "reading totals");
+  HIPCHK(hipStreamSynchronize(s), "count pass");
+  Counters hc = *ctx->hcount;
+  hc.n_entries = ctx->hscalar[0];
+  hc.n_items = ctx->hscalar[1];
+  hc.n_split = ctx->hscalar[2];
+  st = decode_err(ctx, hc.err);
+  if (st != ASOHF_OK) return st;
+  if (hc.n_items >= (1ull << 31)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^31 work items");
+  note_pool(ctx, hc);
+  const int T = a.P.tile[0] * a.P.tile[1] * a.P.tile[2];
+  HIPCHK(ctx->entries.ensure(sizeof(int4) * std::max<unsigned long long>(hc.n_entries, 1)), "allocating entries");
+  HIPCHK(ctx->items.ensure(sizeof(Item) * std::max<unsigned long long>(hc.n_items, 1)), "allocating work items");
+  HIPCHK(ctx->scratch.ensure((size_t)std::max<unsigned long long>(hc.n_split, 1) * T * a.elem),
+         "allocating split-tile scratch");
+  bind_sized(ctx, a);
+  a.grid = (int)std::min<long long>(a.grid, std::max<unsigned long long>(hc.n_items, 1));
+  a.D.n_queues = std::min(kQueues, a.grid);
+  st = enqueue_deposit(ctx, a, s, true);
+  if (st != ASOHF_OK) return st;
+  HIPCHK(hipEventRecord(ctx->ev[6], s), "event");
+  set_stats(ctx, A, hc, true);
+  ctx->sized = true;
+  return ASOHF_OK;
+}
+
+// Batched build.  Once a call has sized the buffers, later calls run without
+// any host round trip before the deposit (the gate kernel checks capacities
+// on the device) and read the counters once at the end; with opts.use_graph
+// the whole sequence is one hipGraph, captured on first use and replayed
+// while every kernel argument stays the same.
+static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, float* u1, float* u11,
+                                double* grid_mass, hipStream_t s) {
+  if (A.n > (1ll << 32)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^32 particles");
+  if (!ctx->sized) return run_exact(ctx, o, A, u1, u11, grid_mass, s);
+  PipeArgs a;
+  asohf_status st = prepare_args(ctx, o, A, u1, u11, grid_mass, a);
+  if (st != ASOHF_OK) return st;
+  bind_sized(ctx, a);
+  hipStream_t rs = s;   // the stream the work runs on
+  bool phases = true;
+  if (o.use_graph) {
+    phases = false;
+    rs = ctx->stream;   // captured and replayed on the library's own stream
+    if (!ctx->gexec || memcmp(&ctx->gkey, &a, sizeof(a)) != 0) {
+      if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+      ctx->gexec = nullptr;
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(rs, hipStreamCaptureModeThreadLocal), "graph capture");
+      st = enqueue_count(ctx, a, rs);
+      if (st == ASOHF_OK) st = enqueue_deposit(ctx, a, rs, false);
+      const hipError_t e = hipStreamEndCapture(rs, &graph);
+      if (st != ASOHF_OK) {
+        if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      HIPCHK(e, "graph capture");
+      const hipError_t ei = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) ctx->gexec = nullptr;
+      HIPCHK(ei, "instantiating the graph");
+      ctx->gkey = a;
+    }
+    if (s != rs) {   // order after the caller's earlier work on its stream
+      HIPCHK(hipEventRecord(ctx->ev[7], s), "event");
+      HIPCHK(hipStreamWaitEvent(rs, ctx->ev[7], 0), "stream wait");
+    }
+    HIPCHK(hipEventRecord(ctx->ev[0], rs), "event");
+    HIPCHK(hipGraphLaunch(ctx->gexec, rs), "graph launch");
+  } else {
+    HIPCHK(hipEventRecord(ctx->ev[0], rs), "event");
+    st = enqueue_count(ctx, a, rs);
+    if (st != ASOHF_OK) return st;
+    HIPCHK(hipEventRecord(ctx->ev[1], rs), "event");
+    st = enqueue_deposit(ctx, a, rs, true);
+    if (st != ASOHF_OK) return st;
+  }
+  HIPCHK(hipEventRecord(ctx->ev[6], rs), "event");
+  HIPCHK(hipMemcpyAsync(ctx->hcount, a.C, sizeof(Counters), hipMemcpyDeviceToHost, rs), "reading counters");
+  HIPCHK(hipStreamSynchronize(rs), "batched build");
+  const Counters hc = *ctx->hcount;
+  st = decode_err(ctx, hc.err);
+  if (st != ASOHF_OK) return st;
+  note_pool(ctx, hc);
+  if (hc.overflow) {
+    // the buffers of earlier calls were too small: nothing was deposited
+    ctx->sized = false;
+    return run_exact(ctx, o, A, u1, u11, grid_mass, s);
+  }
+  set_stats(ctx, A, hc, phases);
   return ASOHF_OK;
 }
 
@@ -781,6 +924,7 @@ void asohf_hip_destroy(asohf_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
   DBuf* bufs[] = {&ctx->pool, &ctx->pool_off, &ctx->pool_nk, &ctx->blk_pairs, &ctx->blk_mmax, &ctx->nsplit,
                   &ctx->item_off, &ctx->split_off, &ctx->plan_blob, &ctx->cnt, &ctx->start, &ctx->cursor, &ctx->bsum, &ctx->counters,
                   &ctx->done, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries, &ctx->items,

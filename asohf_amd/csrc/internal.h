@@ -85,13 +85,22 @@ constexpr int kQueues = 8;          // deposit work queues (one per XCD)
 constexpr int kLine = 32;           // 128 bytes in 4-byte units
 struct Counters {
   unsigned long long n_pairs;     // reduced from per-block partials
+  unsigned long long n_entries;   // totals of the planning scans, copied by the gate kernel
+  unsigned long long n_items;
+  unsigned long long n_split;
   unsigned int err;               // bit 0: non-finite position/mass, bit 1: base index outside
                                   // the single-wrap range, bit 2: negative mass
   unsigned int mmax_bits;         // max particle mass, as FP32 bits (masses >= 0)
-  unsigned int pad[kLine - 4];
+  unsigned int overflow;          // 1: a workspace was smaller than this call needs; the
+                                  // scatter, planning and deposit kernels then do nothing
+  unsigned int n_items_eff;       // items the deposit serves (0 after an overflow)
+  unsigned int pad[kLine - 12];
   unsigned long long pool_fill[kPoolSubs * kLine / 2];   // sub-pool k at [k * kLine / 2]
   unsigned int queue[kQueues * kLine];                   // queue q at [q * kLine]
 };
+
+static_assert(sizeof(unsigned long long) * 4 + sizeof(unsigned) * (kLine - 8) == kLine * 4,
+              "the scalar counters fill one 128-byte line");
 
 // Work item of the persistent deposit: a chunk of one tile's entry list.
 struct Item {
@@ -120,14 +129,12 @@ struct DepArgs {
   const int* split_slot;     // scratch slot of a split tile, -1 otherwise
   unsigned* done;            // finished items per split tile
   void* scratch;             // int64 (deterministic) or FP32 (atomic) tile accumulators
-  Counters* C;
+  Counters* C;               // n_items_eff (the queue length) and mmax_bits (the fixed-point
+                             // scale) are read here, so no host round trip precedes the deposit
   float* u1;
   float* u11;
   long long* tile_mass_i;    // deterministic: exact fixed-point mass per tile
   double* tile_mass_d;       // atomic: FP64 mass per tile
-  double scale;              // fixed-point scale
-  double inv_scale;
-  unsigned n_items;          // items in the queue
   int n_queues;              // queues in use (<= kQueues, <= grid)
 };
 

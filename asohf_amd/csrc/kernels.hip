@@ -219,10 +219,11 @@ __global__ void __launch_bounds__(1024) reduce_blocks(BlockTables BT, long long 
 template <int MODE>
 __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, unsigned long long* cursor,
                                                               int4* entries, unsigned long long cap,
-                                                              BlockTables BT) {
+                                                              BlockTables BT, const Counters* C) {
   __shared__ int keys[kBinCap];
   __shared__ unsigned vals[kBinCap];
   __shared__ unsigned long long base[kBinCap];
+  if (C->overflow) return;   // the entry buffer is too small: the host rebuilds exactly
   for (int k = threadIdx.x; k < kBinCap; k += kBinThreads) { keys[k] = -1; vals[k] = 0u; }
   __syncthreads();
   unsigned err = 0;
@@ -416,9 +417,9 @@ __global__ void plan_count(const unsigned long long* start, int ntiles, int chun
 
 __global__ void plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
                            const unsigned long long* item_off, const unsigned long long* split_off,
-                           Item* items, int* split_slot) {
+                           Item* items, int* split_slot, const Counters* C) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntiles) return;
+  if (t >= ntiles || C->overflow) return;
   long long s0, s1, s2;
   int cn, cw;
   tile_chunks(start, t, chunk, chunk_wide, s0, s1, s2, cn, cw);
@@ -435,6 +436,33 @@ __global__ void plan_write(const unsigned long long* start, int ntiles, int chun
     const long long b = s1 + (long long)c * chunk_wide;
     items[base + k] = Item{t, 1 | (k << 8), b, min(b + (long long)chunk_wide, s2)};
   }
+}
+
+// The totals of the planning scans against the workspaces of this call.  A
+// call whose buffers were sized by an earlier call finds out here, on the
+// device, whether they still suffice; if not, every later kernel of the call
+// does nothing and the host rebuilds with exact sizes.
+__global__ void gate_kernel(const unsigned long long* tot_entries, const unsigned long long* tot_items,
+                            const unsigned long long* tot_split, unsigned long long cap_entries,
+                            unsigned long long cap_items, unsigned long long cap_split, Counters* C) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const unsigned long long ne = *tot_entries, ni = *tot_items, ns = *tot_split;
+  C->n_entries = ne;
+  C->n_items = ni;
+  C->n_split = ns;
+  const bool over = ne > cap_entries || ni > cap_items || ns > cap_split || ni >= (1ull << 31);
+  C->overflow = over ? 1u : 0u;
+  C->n_items_eff = over ? 0u : (unsigned)ni;
+}
+
+// Fixed-point scale of the deterministic reduction: the power of two with
+// max(m) * scale in [2^30, 2^31), so every quantised term fits an int32 and a
+// sum over <= 2^32 particles an int64.
+__device__ __forceinline__ double fixed_scale(unsigned mmax_bits) {
+  const float mm = __uint_as_float(mmax_bits);
+  int e = 0;
+  if (mm > 0.f) (void)frexpf(mm, &e);
+  return ldexp(1.0, 31 - e);
 }
 
 // ---------------------------------------------------------------------------
@@ -537,7 +565,9 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
   __shared__ long long s_redi[16];
   __shared__ double s_redd[16];
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const float scale = (float)D.scale;
+  const double dscale = fixed_scale(D.C->mmax_bits);
+  const float scale = (float)dscale;
+  const unsigned n_items = D.C->n_items_eff;
   const float k0[3] = {P.ktab[P.kmax - 1], P.ktab[P.kmax], P.ktab[P.kmax + 1]};
   // interior cell (a, b, c) of the tile
   auto at = [](int a, int b, int c) { return (a + 2) + HX * ((b + 2) + HY * (c + 2)); };
@@ -550,7 +580,7 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
     if (tid == 0) s_item = atomicAdd(&D.C->queue[q * kLine], 1u) * D.n_queues + q;
     __syncthreads();
     const unsigned idx = __builtin_amdgcn_readfirstlane(s_item);
-    if (idx >= D.n_items) break;
+    if (idx >= n_items) break;
     const Item it = D.items[idx];
     const int t = __builtin_amdgcn_readfirstlane(it.tile);
     const int g = __builtin_amdgcn_readfirstlane(P.tile_grid[t]);
@@ -690,7 +720,7 @@ __global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
       float* out = (g == 0 ? D.u1 : D.u11) + G.off;
       long long msi = 0;
       double msd = 0.0;
-      const double inv_scale = D.inv_scale;
+      const double inv_scale = 1.0 / dscale;   // a power of two: exact
       for (int c = tid; c < T; c += nthr) {
         const int a = c % TX, b = (c / TX) % TY, cz = c / (TX * TY);
         if (a >= ex || b >= ey || cz >= ez) continue;
@@ -726,7 +756,7 @@ __global__ void __launch_bounds__(256) grid_sums(Plan P, DepArgs D, double* grid
     long long s = 0;
     for (int k = threadIdx.x; k < nt; k += blockDim.x) s += D.tile_mass_i[G.tile0 + k];
     s = block_sum(s, shi);
-    if (threadIdx.x == 0) grid_mass[g] = (double)s * D.inv_scale;
+    if (threadIdx.x == 0) grid_mass[g] = (double)s / fixed_scale(D.C->mmax_bits);
   } else {
     double s = 0;
     for (int k = threadIdx.x; k < nt; k += blockDim.x) s += D.tile_mass_d[G.tile0 + k];
@@ -1045,12 +1075,13 @@ hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* 
 }
 
 hipError_t launch_scatter(const Plan& P, const Parts& A, unsigned long long* cursor, int4* entries,
-                          unsigned long long cap, const BlockTables& BT, int cus, hipStream_t s) {
+                          unsigned long long cap, const BlockTables& BT, const Counters* C, int cus,
+                          hipStream_t s) {
   (void)cus;
   if (A.n <= 0) return hipSuccess;
   const int grid = (int)bin_blocks(A.n);
-  if (P.mode == 2) scatter_kernel<2><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap, BT);
-  else scatter_kernel<1><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap, BT);
+  if (P.mode == 2) scatter_kernel<2><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap, BT, C);
+  else scatter_kernel<1><<<grid, 256, 0, s>>>(P, A, cursor, entries, cap, BT, C);
   return hipGetLastError();
 }
 
@@ -1087,9 +1118,16 @@ hipError_t launch_plan_count(const unsigned long long* start, int ntiles, int ch
 
 hipError_t launch_plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
                              const unsigned long long* item_off, const unsigned long long* split_off,
-                             Item* items, int* split_slot, hipStream_t s) {
+                             Item* items, int* split_slot, const Counters* C, hipStream_t s) {
   plan_write<<<grid_for(ntiles, 256, 1 << 30), 256, 0, s>>>(start, ntiles, chunk, chunk_wide, item_off,
-                                                           split_off, items, split_slot);
+                                                           split_off, items, split_slot, C);
+  return hipGetLastError();
+}
+
+hipError_t launch_gate(const unsigned long long* tot_entries, const unsigned long long* tot_items,
+                       const unsigned long long* tot_split, unsigned long long cap_entries,
+                       unsigned long long cap_items, unsigned long long cap_split, Counters* C, hipStream_t s) {
+  gate_kernel<<<1, 64, 0, s>>>(tot_entries, tot_items, tot_split, cap_entries, cap_items, cap_split, C);
   return hipGetLastError();
 }
 

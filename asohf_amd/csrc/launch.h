@@ -11,7 +11,8 @@ long long bin_blocks(long long n);
 hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* C, const BlockTables& BT,
                         int cus, hipStream_t s);
 hipError_t launch_scatter(const Plan& P, const Parts& A, unsigned long long* cursor, int4* entries,
-                          unsigned long long cap, const BlockTables& BT, int cus, hipStream_t s);
+                          unsigned long long cap, const BlockTables& BT, const Counters* C, int cus,
+                          hipStream_t s);
 hipError_t launch_membership(const Plan& P, const Parts& A, int* out, long long cap,
                              unsigned long long* count, int cus, hipStream_t s);
 long long scan_scratch_elems(long long n);
@@ -22,7 +23,10 @@ hipError_t launch_plan_count(const unsigned long long* start, int ntiles, int ch
                              unsigned* nitems_tile, unsigned* nsplit_tile, hipStream_t s);
 hipError_t launch_plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
                              const unsigned long long* item_off, const unsigned long long* split_off,
-                             Item* items, int* split_slot, hipStream_t s);
+                             Item* items, int* split_slot, const Counters* C, hipStream_t s);
+hipError_t launch_gate(const unsigned long long* tot_entries, const unsigned long long* tot_items,
+                       const unsigned long long* tot_split, unsigned long long cap_entries,
+                       unsigned long long cap_items, unsigned long long cap_split, Counters* C, hipStream_t s);
 bool tile_shape_supported(const int t[3]);
 hipError_t launch_deposit(const Plan& P, const DepArgs& D, int reduction, int grid, int block, hipStream_t s);
 hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, double* grid_mass, hipStream_t s);

@@ -99,12 +99,13 @@ class DensityBuilder:
             pass
 
     # ------------------------------------------------------------------ options
-    def _opts(self, mode, reduction="deterministic", path="batched", u1_ld=None, u11_ld=None):
+    def _opts(self, mode, reduction="deterministic", path="batched", u1_ld=None, u11_ld=None,
+              use_graph=False):
         o = _lib.Opts()
         o.mode = _MODES[mode]
         o.reduction = _REDS[reduction]
         o.path = _PATHS[path]
-        o.use_graph = 0
+        o.use_graph = 1 if use_graph else 0
         if u1_ld:
             o.u1_ld[0], o.u1_ld[1] = u1_ld
         if u11_ld:
@@ -122,9 +123,13 @@ class DensityBuilder:
     # ------------------------------------------------------------------ forward
     def density(self, geom: Geometry, x, y, z, m, npart_esp=None, *, mode="table",
                 reduction="deterministic", path="batched", u1=None, u11=None, want_mass=True,
-                u1_ld=None, u11_ld=None) -> DensityResult:
+                grid_mass=None, u1_ld=None, u11_ld=None, use_graph=False) -> DensityResult:
         """U1/U11 of DENSITY (mode "table") or INTERPOLATE_DENSITY_TSC (mode
-        "tsc"), both divided by FDM as RENORM_DENSITY does."""
+        "tsc"), both divided by FDM as RENORM_DENSITY does.
+
+        use_graph: run the batched build as one hipGraph (captured on first use,
+        replayed while the geometry, sizes and every array stay the same; pass
+        u1, u11 and grid_mass to keep the arrays the same)."""
         n = _check_particles(x, y, z, m)
         dev = x.device
         s1, n11 = self.out_shapes(geom, u1_ld, u11_ld)
@@ -132,7 +137,9 @@ class DensityBuilder:
             u1 = torch.empty(s1, dtype=torch.float32, device=dev)
         if u11 is None:
             u11 = torch.empty(n11, dtype=torch.float32, device=dev)
-        gm = torch.empty(1 + geom.npatch_total, dtype=torch.float64, device=dev) if want_mass else None
+        gm = grid_mass
+        if gm is None and want_mass:
+            gm = torch.empty(1 + geom.npatch_total, dtype=torch.float64, device=dev)
         esp = None
         if _MODES[mode] == _lib.MODE_TABLE:
             if npart_esp is None:
@@ -141,7 +148,7 @@ class DensityBuilder:
             if esp.shape != (N_ESP,):
                 raise ValueError("NPART_ESP must have 9 entries")
         ga = _GeomArgs(geom)
-        o = self._opts(mode, reduction, path, u1_ld, u11_ld)
+        o = self._opts(mode, reduction, path, u1_ld, u11_ld, use_graph)
         st = _lib.load().asohf_hip_density(
             self._h, ga.ref, n, _ptr(x), _ptr(y), _ptr(z), _ptr(m),
             _np_ptr(esp, C.c_int32) if esp is not None else None, _ptr(u1), _ptr(u11), _ptr(gm),

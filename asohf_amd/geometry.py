@@ -116,6 +116,17 @@ class Geometry:
     def ncells_total(self) -> int:
         return self.ncells_base + int(self.patch_cells().sum())
 
+    def truncated(self, levels: int) -> "Geometry":
+        """The hierarchy cut to levels 0..levels (the patches of deeper levels
+        dropped), as CREATE_MESH would leave it with NL = levels."""
+        levels = max(0, min(levels, self.nl))
+        k = int(self.npatch[: levels + 1].sum())
+        return Geometry(self.nx, self.ny, self.nz, self.lado0, self.rote, self.rete,
+                        self.npatch[: levels + 1].copy(), self.pnx[:k], self.pny[:k], self.pnz[:k],
+                        self.prx[:k], self.pry[:k], self.prz[:k],
+                        pare=None if self.pare is None else self.pare[:k],
+                        fdm=self.fdm, interp_degree=self.interp_degree)
+
     def denbas(self, ir: int) -> np.float32:
         """DENBAS of level ir in FP32: DXPA*DYPA*DZPA*ROTE*RETE**3 (grids.f:824, 844)."""
         p2 = f32(2.0) ** ir

@@ -449,6 +449,7 @@ def make_snapshot_device(spec: SnapshotSpec, device="cuda:0", chunk: int = 1 << 
     out = [p[order].contiguous() for p in pos]
     del pos, order
     m = torch.full((n,), float(mp), dtype=torch.float32, device=dev)
-    torch.cuda.synchronize(dev)
-    torch.cuda.empty_cache()
+    if dev.type == "cuda":
+        torch.cuda.synchronize(dev)
+        torch.cuda.empty_cache()
     return Snapshot(out[0], out[1], out[2], m, npart_esp, geom, spec)

@@ -13,9 +13,19 @@ Prints ONE JSON line on stdout; progress goes to stderr.
              algorithmic bytes = 16 B per (particle, grid) pair + 4 B per output
              cell, over its average duration measured with HIP events on the
              stream it runs on
+  pipeline   the whole step against the same peak (B_fwd = 16 B per pair + 4 B
+             per cell + 8 B per grid sum; B_min counts each particle once)
+  modes      ms/step of every mode (TSC, kernel table) x reduction
+             (deterministic, atomic) on the same snapshot
+  per_level_ms  builds of the hierarchy cut at level L = 0..NL, differenced
+  adjoint    the hand-written backward (gradients of sum(w * rho))
   plain_gpu  the same build as one launch per grid with global FP32 atomics
   cpu_baseline  the oracle's OpenMP restatement of the reference algorithm
-             (INTERPOLATE_DENSITY_TSC, FP32) on this host's cores
+             (INTERPOLATE_DENSITY_TSC, FP32) on this host's cores; whole for
+             the 2M/16.8M-particle workloads, extrapolated from two particle
+             samples for the 134M/1.07G ones
+Workloads (--config): asohf_amd.synth.CONFIGS; 2 (256^3 particles, 256^3 base
+grid, 4 AMR levels) is the default.
 """
 from __future__ import annotations
 
@@ -49,6 +59,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="seconds of CPU work per sample when the whole CPU build is too long")
+    ap.add_argument("--modes-steps", type=int, default=5, help="steps per mode x reduction (0: skip)")
+    ap.add_argument("--no-levels", dest="levels", action="store_false", help="skip the per-level timing")
+    ap.add_argument("--plain-max-tests", type=float, default=5e12,
+                    help="skip the plain form above this many particle-box tests")
     ap.add_argument("--tune", default="", help="tx,ty,tz,chunk,waves_per_cu: fixed tuning, no autotune")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (counter calibration)")
@@ -80,13 +96,19 @@ def main() -> int:
     from asohf_amd.density import DensityBuilder
 
     spec = synth.CONFIGS[args.config]
-    t0 = time.time()
-    snap = synth.make_snapshot(spec)
-    g = snap.geom
-    log(f"[bench] snapshot {spec.name}: {snap.n} particles, {g.npatch_total} patches, "
-        f"{g.ncells_total} cells, generated in {time.time() - t0:.1f}s")
     dev = torch.device("cuda:0")
-    x, y, z, m = (torch.from_numpy(a).to(dev) for a in (snap.x, snap.y, snap.z, snap.m))
+    t0 = time.time()
+    if spec.n > (1 << 25):
+        # 134M / 1.07G particles: generated on the device (numpy is too slow)
+        snap = synth.make_snapshot_device(spec, device=dev)
+        x, y, z, m = snap.x, snap.y, snap.z, snap.m
+    else:
+        snap = synth.make_snapshot(spec)
+        x, y, z, m = (torch.from_numpy(a).to(dev) for a in (snap.x, snap.y, snap.z, snap.m))
+    n = int(x.numel())
+    g = snap.geom
+    log(f"[bench] snapshot {spec.name}: {n} particles, {g.npatch_total} patches, "
+        f"{g.ncells_total} cells, generated in {time.time() - t0:.1f}s")
     bld = DensityBuilder(0)
     s1, n11 = bld.out_shapes(g)
     u1 = torch.empty(s1, dtype=torch.float32, device=dev)
@@ -115,11 +137,13 @@ def main() -> int:
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = 1000.0 * wall / args.steps
-    value = snap.n / (wall / args.steps)
+    value = n / (wall / args.steps)
     avg = {k: float(np.mean(v)) for k, v in per.items()}
+    med = {k: float(np.median(v)) for k, v in per.items()}
     n_pairs, n_cells, n_entries = st["n_pairs"], st["n_cells"], st["n_entries"]
     P = g.npatch_total
     b_fwd = 16.0 * n_pairs + 4.0 * n_cells + 8.0 * (P + 1)
+    b_min = 16.0 * n + 4.0 * n_cells + 8.0 * (P + 1)
     b_dep = 16.0 * n_pairs + 4.0 * n_cells
     achieved = b_dep / (avg["ms_deposit"] * 1e-3) / 1e9
     log(f"[bench] batched: {ms_step:.3f} ms/step, {value / 1e9:.3f} G particles/s; "
@@ -133,7 +157,7 @@ def main() -> int:
         "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": spec.name, "particles": snap.n, "base_grid": g.nx, "levels": g.nl,
+        "config": {"workload": spec.name, "particles": n, "base_grid": g.nx, "levels": g.nl,
                    "patches": P, "cells": int(n_cells), "mode": args.mode, "reduction": args.reduction,
                    "tune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -141,10 +165,54 @@ def main() -> int:
                      "kernel": "deposit_kernel", "algorithmic_bytes": b_dep,
                      "avg_ms": avg["ms_deposit"]},
         "pipeline": {"algorithmic_bytes": b_fwd, "achieved_gbs": b_fwd / (ms_step * 1e-3) / 1e9,
-                     "frac": b_fwd / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, "phase_ms": avg,
+                     "frac": b_fwd / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "b_min": b_min, "b_min_frac": b_min / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "phase_ms": avg, "phase_ms_median": med,
                      "n_pairs": int(n_pairs), "n_entries": int(n_entries)},
     }
     u1_gpu = r.u1.cpu().numpy()
+
+    if args.modes_steps > 0:
+        # every mode x reduction on the same snapshot (the headline line above is one of them)
+        table = {}
+        for md in ("tsc", "table"):
+            for rd in ("deterministic", "atomic"):
+                k2 = dict(kw, mode=md, reduction=rd)
+                bld.density(g, x, y, z, m, snap.npart_esp, **k2)
+                torch.cuda.synchronize()
+                dd = []
+                t0 = time.perf_counter()
+                for _ in range(args.modes_steps):
+                    bld.density(g, x, y, z, m, snap.npart_esp, **k2)
+                    dd.append(bld.stats()["ms_deposit"])
+                torch.cuda.synchronize()
+                tw = (time.perf_counter() - t0) / args.modes_steps
+                s2 = bld.stats()
+                table[f"{md}/{rd}"] = {"ms_per_step": 1000 * tw, "particles_per_s": n / tw,
+                                      "deposit_ms": float(np.mean(dd)), "n_entries": int(s2["n_entries"])}
+        out["modes"] = table
+        log(f"[bench] modes: {table}")
+        bld.density(g, x, y, z, m, snap.npart_esp, **kw)
+        torch.cuda.synchronize()
+
+    if args.levels and g.nl > 0:
+        # time per AMR level: builds of the hierarchy cut at level L, differenced
+        cum = []
+        for L in range(0, g.nl + 1):
+            gl = g.truncated(L)
+            bld.density(gl, x, y, z, m, snap.npart_esp, mode=args.mode, reduction=args.reduction)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(max(args.steps // 4, 3)):
+                bld.density(gl, x, y, z, m, snap.npart_esp, mode=args.mode, reduction=args.reduction)
+            torch.cuda.synchronize()
+            cum.append(1000 * (time.perf_counter() - t0) / max(args.steps // 4, 3))
+        out["per_level_ms"] = {"cumulative": cum,
+                               "level": [cum[0]] + [cum[i] - cum[i - 1] for i in range(1, len(cum))],
+                               "patches": [int(v) for v in g.npatch]}
+        log(f"[bench] per level (cumulative ms): {cum}")
+        bld.density(g, x, y, z, m, snap.npart_esp, **kw)
+        torch.cuda.synchronize()
 
     if args.mode == "tsc":
         # the hand-written adjoint: d(sum(w*rho))/d(positions, masses)
@@ -158,45 +226,107 @@ def main() -> int:
             bld.adjoint(g, x, y, z, m, w1, w11)
         torch.cuda.synchronize()
         aw = 1000.0 * (time.perf_counter() - t0) / args.steps
-        b_adj = 16.0 * n_pairs + 4.0 * n_cells + 16.0 * snap.n
+        b_adj = 16.0 * n_pairs + 4.0 * n_cells + 16.0 * n
         out["adjoint"] = {"ms_per_step": aw, "ratio_to_forward": aw / ms_step,
                           "algorithmic_bytes": b_adj, "achieved_gbs": b_adj / (aw * 1e-3) / 1e9}
         log(f"[bench] adjoint: {aw:.3f} ms/step ({aw / ms_step:.2f}x forward)")
+        del w1, w11
 
     if not args.no_plain:
         kwp = dict(kw, path="plain", reduction="atomic")
-        bld.density(g, x, y, z, m, snap.npart_esp, **kwp)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.plain_steps):
+        scan = float(P) * n        # particle-box tests of the one-launch-per-patch form
+        if scan > args.plain_max_tests:
+            out["plain_gpu"] = {"skipped": f"{P} patch launches over {n} particles "
+                                           f"({scan:.1e} particle-box tests) exceed the bench's bound"}
+        else:
+            steps_p = args.plain_steps if scan < 1e11 else 1
             bld.density(g, x, y, z, m, snap.npart_esp, **kwp)
-        torch.cuda.synchronize()
-        pw = (time.perf_counter() - t0) / args.plain_steps
-        out["plain_gpu"] = {"value": snap.n / pw, "unit": "particles/s", "ms_per_step": 1000 * pw,
-                            "launches": 2 + P, "speedup_batched_vs_plain": (snap.n / (wall / args.steps)) / (snap.n / pw)}
-        log(f"[bench] plain: {1000 * pw:.3f} ms/step")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps_p):
+                bld.density(g, x, y, z, m, snap.npart_esp, **kwp)
+            torch.cuda.synchronize()
+            pw = (time.perf_counter() - t0) / steps_p
+            out["plain_gpu"] = {"value": n / pw, "unit": "particles/s", "ms_per_step": 1000 * pw,
+                                "steps": steps_p, "launches": 2 + P,
+                                "speedup_batched_vs_plain": value / (n / pw)}
+            log(f"[bench] plain: {1000 * pw:.3f} ms/step")
 
     if not args.no_cpu:
-        # cpu_baseline leg: the oracle's restatement of the reference algorithm
-        from oracle import oracle as O
-        cores = args.cpu_threads or len(os.sched_getaffinity(0))
-        os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-        mode = "B" if args.mode == "tsc" else "A"
-        t0 = time.perf_counter()
-        c1, c11 = O.density32(snap, mode, nthreads=cores)
-        tc = time.perf_counter() - t0
-        sel = c1 > c1.mean()
-        diff = float(np.max(np.abs(u1_gpu[sel] - c1[sel]) / c1[sel]))
-        out["cpu_baseline"] = {"value": snap.n / tc, "unit": "particles/s", "cores": cores,
-                               "kind": "port",
-                               "sample": f"the whole {spec.name} build ({snap.n} particles, {P} patches) "
-                                         f"by the FP32 OpenMP restatement of "
-                                         f"{'INTERPOLATE_DENSITY_TSC' if mode == 'B' else 'DENSITY'}",
-                               "seconds": tc, "max_rel_diff_base_vs_gpu": diff}
-        log(f"[bench] cpu baseline: {tc:.2f}s on {cores} threads")
+        out["cpu_baseline"] = cpu_baseline(args, snap, (x, y, z, m), u1_gpu, n, P)
     print(json.dumps(out), flush=True)
     bld.close()
     return 0
+
+
+def cpu_baseline(args, snap, dev_xyzm, u1_gpu, n, P):
+    """The oracle's OpenMP restatement of the reference build on this host.
+
+    Small workloads are built whole. For the large ones (whose CPU build is
+    hours: every particle is tested against every patch box of a level), two
+    seeded particle samples of the same snapshot are built with the full
+    hierarchy, T(k) = T0 + k * t is fitted through them, and the rate is
+    N / (T0 + N * t), labelled as extrapolated."""
+    import numpy as np
+    import types
+
+    from oracle import oracle as O
+
+    cores = args.cpu_threads or len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    mode = "B" if args.mode == "tsc" else "A"
+    name = "INTERPOLATE_DENSITY_TSC" if mode == "B" else "DENSITY"
+    x, y, z, m = dev_xyzm
+    if n <= (1 << 25):
+        host = [a if isinstance(a, np.ndarray) else a.cpu().numpy() for a in (snap.x, snap.y, snap.z, snap.m)]
+        sn = types.SimpleNamespace(x=host[0], y=host[1], z=host[2], m=host[3],
+                                   npart_esp=snap.npart_esp, geom=snap.geom)
+        t0 = time.perf_counter()
+        c1, _ = O.density32(sn, mode, nthreads=cores)
+        tc = time.perf_counter() - t0
+        sel = c1 > c1.mean()
+        diff = float(np.max(np.abs(u1_gpu[sel] - c1[sel]) / c1[sel])) if u1_gpu is not None else None
+        log(f"[bench] cpu baseline: {tc:.2f}s on {cores} threads")
+        return {"value": n / tc, "unit": "particles/s", "cores": cores, "kind": "port",
+                "sample": f"the whole build ({n} particles, {P} patches) by the FP32 OpenMP "
+                          f"restatement of {name}",
+                "seconds": tc, "max_rel_diff_base_vs_gpu": diff}
+
+    import torch
+    rng = np.random.default_rng(snap.spec.seed if snap.spec else 0)
+    bounds = np.concatenate([[0], np.cumsum(snap.npart_esp.astype(np.int64))])
+
+    def sample(k):
+        idx = np.sort(rng.choice(n, size=k, replace=False))
+        ti = torch.from_numpy(idx).to(x.device)
+        arrs = [a[ti].cpu().numpy() for a in (x, y, z, m)]
+        esp = np.bincount(np.searchsorted(bounds, idx, side="right") - 1, minlength=len(snap.npart_esp))
+        return types.SimpleNamespace(x=arrs[0], y=arrs[1], z=arrs[2], m=arrs[3],
+                                     npart_esp=esp[: len(snap.npart_esp)].astype(np.int32), geom=snap.geom)
+
+    def timed(sn):
+        t0 = time.perf_counter()
+        O.density32(sn, mode, nthreads=cores)
+        return time.perf_counter() - t0
+
+    k0 = 4096
+    t_probe = timed(sample(k0))
+    ka = int(min(max(k0, k0 * args.cpu_budget / 2 / max(t_probe, 1e-3)), n // 4))
+    ta = timed(sample(ka))
+    tb = timed(sample(2 * ka))
+    slope = (tb - ta) / ka
+    fixed = max(ta - ka * slope, 0.0)
+    if slope <= 0:
+        slope, fixed = tb / (2 * ka), 0.0
+    t_full = fixed + n * slope
+    log(f"[bench] cpu baseline: samples {ka}/{2 * ka} particles took {ta:.2f}/{tb:.2f}s on {cores} "
+        f"threads; extrapolated whole build {t_full:.0f}s")
+    return {"value": n / t_full, "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": f"extrapolated: two seeded samples of {ka} and {2 * ka} of the {n} particles, "
+                      f"each built with the full {P}-patch hierarchy by the FP32 OpenMP restatement "
+                      f"of {name}; time fitted as T0 + k*t and evaluated at k = {n}",
+            "seconds_samples": [ta, tb], "seconds_extrapolated": t_full, "fixed_s": fixed,
+            "per_particle_s": slope}
 
 
 if __name__ == "__main__":
