@@ -122,9 +122,9 @@ struct HostPlan {
   bool valid = false;
   std::vector<GridDesc> grids;
   std::vector<int> tile_grid;
-  std::vector<uint64_t> hkeys;
-  std::vector<int2> hvals;
-  std::vector<int> hlist;
+  std::vector<HashSlot> hslots;
+  std::vector<BoxRec> hrec;
+  std::vector<int> hlist;    // grid id of every candidate (host copy, for the nesting check)
   std::vector<float> radx, rady, radz, ktab;
   Plan dev{};
   long long ncells = 0;
@@ -434,13 +434,11 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   }
   size_t cap = 16;
   while (cap < 2 * ukeys.size() + 2) cap <<= 1;
-  H.hkeys.assign(cap, kHashEmpty);
-  H.hvals.assign(cap, make_int2(0, 0));
+  H.hslots.assign(cap, HashSlot{kHashEmpty, 0, 0});
   for (size_t i = 0; i < ukeys.size(); ++i) {
     unsigned s = (unsigned)hash_mix(ukeys[i]) & (unsigned)(cap - 1);
-    while (H.hkeys[s] != kHashEmpty) s = (s + 1) & (unsigned)(cap - 1);
-    H.hkeys[s] = ukeys[i];
-    H.hvals[s] = uvals[i];
+    while (H.hslots[s].key != kHashEmpty) s = (s + 1) & (unsigned)(cap - 1);
+    H.hslots[s] = HashSlot{ukeys[i], uvals[i].x, uvals[i].y};
   }
   // nesting: does every level-l membership box lie inside some level-(l-1)
   // box (FP32 values, as the device compares them)?  Then a particle outside
@@ -470,14 +468,21 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
     }
   }
   if (H.hlist.empty()) H.hlist.push_back(0);
+  H.hrec.resize(H.hlist.size());
+  for (size_t i = 0; i < H.hlist.size(); ++i) {
+    const GridDesc& G = H.grids[H.hlist[i]];
+    BoxRec& r = H.hrec[i];
+    for (int a = 0; a < 3; ++a) { r.lo[a] = G.lo[a]; r.hi[a] = G.hi[a]; }
+    r.g = H.hlist[i];
+    r.pad = 0;
+  }
   // ---- upload ----
   size_t off = 0;
   auto place = [&](size_t bytes) { size_t o2 = off; off = align_up(off + bytes, 256); return o2; };
   const size_t o_grids = place(sizeof(GridDesc) * H.grids.size());
   const size_t o_tg = place(sizeof(int) * std::max<size_t>(H.tile_grid.size(), 1));
-  const size_t o_hk = place(sizeof(uint64_t) * H.hkeys.size());
-  const size_t o_hv = place(sizeof(int2) * H.hvals.size());
-  const size_t o_hl = place(sizeof(int) * H.hlist.size());
+  const size_t o_hs = place(sizeof(HashSlot) * H.hslots.size());
+  const size_t o_hr = place(sizeof(BoxRec) * H.hrec.size());
   const size_t o_rx = place(sizeof(float) * H.radx.size());
   const size_t o_ry = place(sizeof(float) * H.rady.size());
   const size_t o_rz = place(sizeof(float) * H.radz.size());
@@ -489,9 +494,8 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   };
   HIPCHK(up(o_grids, H.grids.data(), sizeof(GridDesc) * H.grids.size()), "uploading grids");
   if (!H.tile_grid.empty()) HIPCHK(up(o_tg, H.tile_grid.data(), sizeof(int) * H.tile_grid.size()), "uploading tiles");
-  HIPCHK(up(o_hk, H.hkeys.data(), sizeof(uint64_t) * H.hkeys.size()), "uploading hash");
-  HIPCHK(up(o_hv, H.hvals.data(), sizeof(int2) * H.hvals.size()), "uploading hash");
-  HIPCHK(up(o_hl, H.hlist.data(), sizeof(int) * H.hlist.size()), "uploading hash");
+  HIPCHK(up(o_hs, H.hslots.data(), sizeof(HashSlot) * H.hslots.size()), "uploading hash");
+  HIPCHK(up(o_hr, H.hrec.data(), sizeof(BoxRec) * H.hrec.size()), "uploading hash");
   HIPCHK(up(o_rx, H.radx.data(), sizeof(float) * H.radx.size()), "uploading RADX");
   HIPCHK(up(o_ry, H.rady.data(), sizeof(float) * H.rady.size()), "uploading RADY");
   HIPCHK(up(o_rz, H.radz.data(), sizeof(float) * H.radz.size()), "uploading RADZ");
@@ -499,9 +503,8 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   HIPCHK(hipStreamSynchronize(ctx->stream), "uploading the plan");
   D.grids = (const GridDesc*)(base + o_grids);
   D.tile_grid = (const int*)(base + o_tg);
-  D.hkeys = (const uint64_t*)(base + o_hk);
-  D.hvals = (const int2*)(base + o_hv);
-  D.hlist = (const int*)(base + o_hl);
+  D.hslots = (const HashSlot*)(base + o_hs);
+  D.hrec = (const BoxRec*)(base + o_hr);
   D.radx = (const float*)(base + o_rx);
   D.rady = (const float*)(base + o_ry);
   D.radz = (const float*)(base + o_rz);

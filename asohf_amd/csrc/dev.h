@@ -88,11 +88,16 @@ __device__ __forceinline__ int2 bucket_lookup(const Plan& P, int l, float x, flo
   unsigned s = (unsigned)hash_mix(key) & P.hmask;
   // the table is at most half full, so an empty slot ends every probe
   for (;;) {
-    uint64_t k = P.hkeys[s];
-    if (k == key) return P.hvals[s];
-    if (k == kHashEmpty) return make_int2(0, 0);
+    const HashSlot h = P.hslots[s];
+    if (h.key == key) return make_int2(h.start, h.count);
+    if (h.key == kHashEmpty) return make_int2(0, 0);
     s = (s + 1) & P.hmask;
   }
+}
+
+// strict membership test of the reference's box rule on a candidate record
+__device__ __forceinline__ bool in_box(const BoxRec& r, float x, float y, float z) {
+  return r.lo[0] < x && x < r.hi[0] && r.lo[1] < y && y < r.hi[1] && r.lo[2] < z && z < r.hi[2];
 }
 
 __device__ __forceinline__ int species_of(const Parts& A, long long i) {
@@ -183,10 +188,10 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
     const int2 se = bucket_lookup(P, l, x, y, z);
     bool any = false;
     for (int q = 0; q < se.y; ++q) {
-      const int g = P.hlist[se.x + q];
+      const BoxRec r = P.hrec[se.x + q];
+      if (!in_box(r, x, y, z)) continue;
+      const int g = r.g;
       const GridDesc& G = P.grids[g];
-      if (!(G.lo[0] < x && x < G.hi[0] && G.lo[1] < y && y < G.hi[1] && G.lo[2] < z && z < G.hi[2]))
-        continue;
       any = true;
       int lo[3], hi[3], ci[3];
       int irk = 0;

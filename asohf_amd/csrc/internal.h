@@ -32,6 +32,23 @@ struct GridDesc {
   long long off;     // output offset of cell (1,1,1) in u1 (g = 0) or u11
 };
 
+// Slot of the per-level spatial hash: bucket key and its candidate range, in
+// one 16-byte load.
+struct HashSlot {
+  uint64_t key;              // kHashEmpty for a free slot
+  int start;                 // candidates hrec[start, start + count)
+  int count;
+};
+
+// A candidate patch of a bucket: its membership box inline, so the box test
+// needs no load of the grid descriptor.
+struct BoxRec {
+  float lo[3];
+  float hi[3];
+  int g;                     // grid id (1 + patch)
+  int pad;
+};
+
 // Per-level spatial hash of patch membership boxes.
 struct LevelHash {
   float x0, inv_bs;  // bucket b = clamp(floor((x - x0) * inv_bs), 0, nb-1)
@@ -43,9 +60,8 @@ struct LevelHash {
 struct Plan {
   const GridDesc* grids;     // [1 + P]
   const int* tile_grid;      // [ntiles]
-  const uint64_t* hkeys;     // [hcap]
-  const int2* hvals;         // [hcap]  (start, count) into hlist
-  const int* hlist;          // grid ids
+  const HashSlot* hslots;    // [hcap] open addressing, at most half full
+  const BoxRec* hrec;        // candidates, bucket by bucket
   const float* radx;         // RADX(0:NX+1) etc.
   const float* rady;
   const float* radz;

@@ -811,11 +811,10 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
     for (int l = 1; l <= P.nl; ++l) {
       const int2 se = bucket_lookup(P, l, p[0], p[1], p[2]);
       for (int q = 0; q < se.y; ++q) {
-        const int g = P.hlist[se.x + q];
+        const BoxRec r = P.hrec[se.x + q];
+        if (!in_box(r, p[0], p[1], p[2])) continue;
+        const int g = r.g;
         const GridDesc& G = P.grids[g];
-        if (!(G.lo[0] < p[0] && p[0] < G.hi[0] && G.lo[1] < p[1] && p[1] < G.hi[1] && G.lo[2] < p[2] &&
-              p[2] < G.hi[2]))
-          continue;
         int ci[3];
         float w[3][3], dw[3][3];
 #pragma unroll
@@ -949,10 +948,8 @@ __global__ void __launch_bounds__(256) plain_level_flags(Plan P, Parts A, int l,
     const float x = A.x[i], y = A.y[i], z = A.z[i];
     unsigned f = 0;
     const int2 se = bucket_lookup(P, l, x, y, z);
-    for (int q = 0; q < se.y && !f; ++q) {
-      const GridDesc& G = P.grids[P.hlist[se.x + q]];
-      if (G.lo[0] < x && x < G.hi[0] && G.lo[1] < y && y < G.hi[1] && G.lo[2] < z && z < G.hi[2]) f = 1;
-    }
+    for (int q = 0; q < se.y && !f; ++q)
+      if (in_box(P.hrec[se.x + q], x, y, z)) f = 1;
     flag[i] = f;
   }
 }
