@@ -1082,8 +1082,8 @@ asohf_status asohf_hip_get_tune(const asohf_ctx* ctx, asohf_tune* out) {
 asohf_status asohf_hip_set_tune(asohf_ctx* ctx, const asohf_tune* t) {
   if (!ctx || !t) return ASOHF_ERR_INVALID;
   if (!tile_shape_supported(t->tile)) return fail(ctx, ASOHF_ERR_INVALID, "unsupported tile shape");
-  if (t->block < 64 || t->block > 256 || t->block % 64)
-    return fail(ctx, ASOHF_ERR_INVALID, "block must be 64, 128, 192 or 256 threads");
+  if (t->block < 64 || t->block > 1024 || t->block % 64)
+    return fail(ctx, ASOHF_ERR_INVALID, "block must be a multiple of 64 threads, at most 1024");
   if (t->chunk < 64 || t->waves_per_cu < 1 || t->waves_per_cu > 64)
     return fail(ctx, ASOHF_ERR_INVALID, "chunk >= 64 and 1 <= waves_per_cu <= 64");
   const bool reshape = memcmp(t->tile, ctx->tune.tile, sizeof(t->tile)) != 0;
@@ -1098,28 +1098,50 @@ asohf_status asohf_hip_autotune(asohf_ctx* ctx, const asohf_geom* geom, int64_t 
   if (!ctx) return ASOHF_ERR_INVALID;
   asohf_opts o = opts ? *opts : default_opts();
   o.path = ASOHF_PATH_BATCHED;
+  // coordinate descent: tile shape, then workgroup size, then item chunk,
+  // then persistent grid; each candidate is one warm-up call and two timed ones
   static const int shapes[][3] = {{8, 8, 8}, {16, 8, 8}, {16, 16, 8}, {16, 16, 16}};
-  static const int chunks[] = {2048, 8192};
+  static const int blocks[] = {256, 512, 1024};
+  static const int chunks[] = {2048, 8192, 32768};
   static const int waves[] = {8, 16, 32};
   asohf_tune best = ctx->tune;
   double best_ms = 1e30;
-  for (auto& sh : shapes)
-    for (int ch : chunks)
-      for (int wv : waves) {
-        asohf_tune t{{sh[0], sh[1], sh[2]}, 256, ch, wv};
-        asohf_status st = asohf_hip_set_tune(ctx, &t);
-        if (st != ASOHF_OK) return st;
-        double ms = 0;
-        for (int rep = 0; rep < 3; ++rep) {
-          st = asohf_hip_density(ctx, geom, n, rx, ry, rz, m, npart_esp, u1, u11, nullptr, &o);
-          if (st != ASOHF_OK) return st;
-          asohf_stats stt;
-          st = asohf_hip_get_stats(ctx, &stt);
-          if (st != ASOHF_OK) return st;
-          if (rep > 0) ms += stt.ms_total;
-        }
-        if (ms < best_ms) { best_ms = ms; best = t; }
-      }
+  auto trial = [&](const asohf_tune& t) -> asohf_status {
+    asohf_status st = asohf_hip_set_tune(ctx, &t);
+    if (st != ASOHF_OK) return st;
+    double ms = 0;
+    for (int rep = 0; rep < 3; ++rep) {
+      st = asohf_hip_density(ctx, geom, n, rx, ry, rz, m, npart_esp, u1, u11, nullptr, &o);
+      if (st != ASOHF_OK) return st;
+      asohf_stats stt;
+      st = asohf_hip_get_stats(ctx, &stt);
+      if (st != ASOHF_OK) return st;
+      if (rep > 0) ms += stt.ms_total;
+    }
+    if (ms < best_ms) { best_ms = ms; best = t; }
+    return ASOHF_OK;
+  };
+  asohf_status st0;
+  for (auto& sh : shapes) {
+    asohf_tune t = best;
+    t.tile[0] = sh[0]; t.tile[1] = sh[1]; t.tile[2] = sh[2];
+    if ((st0 = trial(t)) != ASOHF_OK) return st0;
+  }
+  for (int bl : blocks) {
+    asohf_tune t = best;
+    t.block = bl;
+    if ((st0 = trial(t)) != ASOHF_OK) return st0;
+  }
+  for (int ch : chunks) {
+    asohf_tune t = best;
+    t.chunk = ch;
+    if ((st0 = trial(t)) != ASOHF_OK) return st0;
+  }
+  for (int wv : waves) {
+    asohf_tune t = best;
+    t.waves_per_cu = wv;
+    if ((st0 = trial(t)) != ASOHF_OK) return st0;
+  }
   asohf_status st = asohf_hip_set_tune(ctx, &best);
   if (st != ASOHF_OK) return st;
   if (out) *out = best;

@@ -555,7 +555,7 @@ __device__ __forceinline__ int centre_local(int c, int org, int ext, int n, bool
 }
 
 template <int MODE, int RED, int TX, int TY, int TZ>
-__global__ void __launch_bounds__(256) deposit_kernel(Plan P, DepArgs D) {
+__global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
   using Acc = typename AccT<RED>::T;
   constexpr int T = TX * TY * TZ;
   constexpr int HX = TX + 4, HY = TY + 4, HZ = TZ + 4, H = HX * HY * HZ;

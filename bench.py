@@ -65,7 +65,7 @@ def main() -> int:
     ap.add_argument("--no-levels", dest="levels", action="store_false", help="skip the per-level timing")
     ap.add_argument("--plain-max-tests", type=float, default=5e12,
                     help="skip the plain form above this many particle-box tests")
-    ap.add_argument("--tune", default="", help="tx,ty,tz,chunk,waves_per_cu: fixed tuning, no autotune")
+    ap.add_argument("--tune", default="", help="tx,ty,tz,chunk,waves_per_cu[,block]: fixed tuning, no autotune")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (counter calibration)")
     args = ap.parse_args()
@@ -116,8 +116,9 @@ def main() -> int:
     kw = dict(mode=args.mode, reduction=args.reduction, u1=u1, u11=u11)
     tune = bld.get_tune()
     if args.tune:
-        tx, ty, tz, ch, wv = (int(v) for v in args.tune.split(","))
-        bld.set_tune(tile=(tx, ty, tz), chunk=ch, waves_per_cu=wv)
+        tv = [int(v) for v in args.tune.split(",")]
+        tx, ty, tz, ch, wv = tv[:5]
+        bld.set_tune(tile=(tx, ty, tz), chunk=ch, waves_per_cu=wv, block=tv[5] if len(tv) > 5 else 256)
         tune = bld.get_tune()
     elif not args.no_autotune:
         t0 = time.time()
