@@ -58,11 +58,10 @@ class Stats(C.Structure):
     _fields_ = [("n_particles", C.c_int64), ("n_pairs", C.c_int64), ("n_entries", C.c_int64),
                 ("n_tiles", C.c_int64), ("n_items", C.c_int64), ("n_cells", C.c_int64),
                 ("ms_total", C.c_double), ("ms_bin", C.c_double), ("ms_scatter", C.c_double),
-                ("ms_deposit", C.c_double), ("ms_sums", C.c_double),
-                ("level_min", C.c_float * 10), ("level_max", C.c_float * 10)]
+                ("ms_deposit", C.c_double), ("ms_sums", C.c_double), ("ms_split", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k not in ("level_min", "level_max")}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class AsohfError(RuntimeError):

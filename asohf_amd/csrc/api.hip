@@ -150,7 +150,9 @@ struct PipeArgs {
   unsigned long long* item_off;
   unsigned long long* split_off;
   int* split;
-  unsigned* done;
+  unsigned* is_split;
+  unsigned long long* split_rank;
+  int* split_tiles;
   double* grid_mass;
   long long nblk;
   long long nkeys;
@@ -168,7 +170,7 @@ struct asohf_ctx {
   asohf_tune tune{{16, 8, 8}, 256, 4096, 16};
   HostPlan plan;
   DBuf plan_blob;
-  DBuf cnt, start, cursor, bsum, counters, done, nitems, split, tmass, entries, items, scratch;
+  DBuf cnt, start, cursor, bsum, counters, stiles, isplit, srank, nitems, split, tmass, entries, items, scratch;
   DBuf flags, idx, pool, pool_off, pool_nk, blk_pairs, blk_mmax, nsplit, item_off, split_off;
   long long pool_want = 0;   // pairs the last count pass needed
   bool sized = false;        // the entry/item/scratch buffers were sized by an exact call
@@ -177,8 +179,9 @@ struct asohf_ctx {
   DBuf h_in[4], h_out[2], h_mass;    // device copies for the host entry point
   Counters* hcount = nullptr;          // pinned
   unsigned long long* hscalar = nullptr;
-  hipEvent_t ev[8] = {};   // 0 start, 1 count pass, 2 scatter start, 3 scatter end, 4/5 deposit, 6 end,
-                           // 7 the caller's stream before a graph launch
+  hipEvent_t ev[9] = {};   // 0 start, 1 count pass, 2 scatter start, 3 scatter end, 4 deposit start,
+                           // 8 deposit end, 5 split tiles summed, 6 end, 7 the caller's stream
+                           // before a graph launch
   asohf_stats stats{};
   bool stats_pending = false;
   bool stats_phases = false;
@@ -612,7 +615,9 @@ static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Part
   HIPCHK(ctx->cursor.ensure(sizeof(unsigned long long) * std::max<long long>(nkeys, 1)), "allocating cursors");
   HIPCHK(ctx->bsum.ensure(sizeof(unsigned long long) * scan_scratch_elems(nkeys)), "allocating scan");
   HIPCHK(ctx->counters.ensure(sizeof(Counters)), "allocating counters");
-  HIPCHK(ctx->done.ensure(sizeof(unsigned) * std::max(ntiles, 1)), "allocating tile state");
+  HIPCHK(ctx->stiles.ensure(sizeof(int) * std::max(ntiles, 1)), "allocating tile state");
+  HIPCHK(ctx->isplit.ensure(sizeof(unsigned) * std::max(ntiles, 1)), "allocating tile state");
+  HIPCHK(ctx->srank.ensure(sizeof(unsigned long long) * (ntiles + 1)), "allocating tile state");
   HIPCHK(ctx->nitems.ensure(sizeof(unsigned) * std::max(ntiles, 1)), "allocating tile state");
   HIPCHK(ctx->nsplit.ensure(sizeof(unsigned) * std::max(ntiles, 1)), "allocating tile state");
   HIPCHK(ctx->item_off.ensure(sizeof(unsigned long long) * (ntiles + 1)), "allocating tile state");
@@ -643,7 +648,9 @@ static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Part
   a.item_off = ctx->item_off.as<unsigned long long>();
   a.split_off = ctx->split_off.as<unsigned long long>();
   a.split = ctx->split.as<int>();
-  a.done = ctx->done.as<unsigned>();
+  a.is_split = ctx->isplit.as<unsigned>();
+  a.split_rank = ctx->srank.as<unsigned long long>();
+  a.split_tiles = ctx->stiles.as<int>();
   a.grid_mass = grid_mass;
   a.chunk = std::max(ctx->tune.chunk, 64);
   a.chunk_wide = std::max(a.chunk / 64, 4);
@@ -653,7 +660,8 @@ static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Part
   a.grid = std::max(1, ctx->cus * waves / std::max(a.block / 64, 1));
   a.D.nitems_tile = ctx->nitems.as<int>();
   a.D.split_slot = ctx->split.as<int>();
-  a.D.done = ctx->done.as<unsigned>();
+  a.D.split_tiles = ctx->stiles.as<int>();
+  a.D.n_split_tiles = ctx->srank.as<unsigned long long>() + ntiles;
   a.D.C = a.C;
   a.D.u1 = u1;
   a.D.u11 = u11;
@@ -678,13 +686,14 @@ static void bind_sized(asohf_ctx* ctx, PipeArgs& a) {
 static asohf_status enqueue_count(asohf_ctx* ctx, const PipeArgs& a, hipStream_t s) {
   HIPCHK(launch_fill_u32((unsigned*)a.C, 0u, sizeof(Counters) / 4, s), "clearing counters");
   HIPCHK(launch_fill_u32(a.cnt, 0u, a.nkeys, s), "clearing counts");
-  HIPCHK(launch_fill_u32(a.done, 0u, a.ntiles, s), "clearing tile state");
   HIPCHK(launch_count(a.P, a.A, a.cnt, a.C, a.BT, a.cus, s), "count kernel");
   if (a.nblk > 0) HIPCHK(launch_reduce_blocks(a.BT, a.nblk, a.C, s), "block sums");
   HIPCHK(launch_scan(a.cnt, a.nkeys, a.bsum, a.start, a.cursor, s), "scan");
-  HIPCHK(launch_plan_count(a.start, a.ntiles, a.chunk, a.chunk_wide, a.nitems, a.nsplit, s), "planning items");
+  HIPCHK(launch_plan_count(a.start, a.ntiles, a.chunk, a.chunk_wide, a.nitems, a.nsplit, a.is_split, s),
+         "planning items");
   HIPCHK(launch_scan(a.nitems, a.ntiles, a.bsum, a.item_off, nullptr, s), "item offsets");
   HIPCHK(launch_scan(a.nsplit, a.ntiles, a.bsum, a.split_off, nullptr, s), "split offsets");
+  HIPCHK(launch_scan(a.is_split, a.ntiles, a.bsum, a.split_rank, nullptr, s), "split tiles");
   return ASOHF_OK;
 }
 
@@ -696,10 +705,13 @@ static asohf_status enqueue_deposit(asohf_ctx* ctx, const PipeArgs& a, hipStream
   if (events) HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
   HIPCHK(launch_scatter(a.P, a.A, a.cursor, const_cast<int4*>(a.D.entries), a.cap_entries, a.BT, a.C, a.cus, s), "scatter kernel");
   if (events) HIPCHK(hipEventRecord(ctx->ev[3], s), "event");
-  HIPCHK(launch_plan_write(a.start, a.ntiles, a.chunk, a.chunk_wide, a.item_off, a.split_off,
-                           const_cast<Item*>(a.D.items), a.split, a.C, s), "writing items");
+  HIPCHK(launch_plan_write(a.start, a.ntiles, a.chunk, a.chunk_wide, a.item_off, a.split_off, a.split_rank,
+                           const_cast<Item*>(a.D.items), a.split, a.split_tiles, a.D.tile_mass_i, a.C, s),
+         "writing items");
   if (events) HIPCHK(hipEventRecord(ctx->ev[4], s), "event");
   HIPCHK(launch_deposit(a.P, a.D, a.red, a.grid, a.block, s), "deposit kernel");
+  if (events) HIPCHK(hipEventRecord(ctx->ev[8], s), "event");
+  HIPCHK(launch_split_reduce(a.P, a.D, a.red, a.cus, s), "split tiles");
   if (events) HIPCHK(hipEventRecord(ctx->ev[5], s), "event");
   if (a.grid_mass) HIPCHK(launch_grid_sums(a.P, a.D, a.red, a.grid_mass, s), "grid sums");
   return ASOHF_OK;
@@ -930,7 +942,7 @@ void asohf_hip_destroy(asohf_ctx* ctx) {
   if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
   DBuf* bufs[] = {&ctx->pool, &ctx->pool_off, &ctx->pool_nk, &ctx->blk_pairs, &ctx->blk_mmax, &ctx->nsplit,
                   &ctx->item_off, &ctx->split_off, &ctx->plan_blob, &ctx->cnt, &ctx->start, &ctx->cursor, &ctx->bsum, &ctx->counters,
-                  &ctx->done, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries, &ctx->items,
+                  &ctx->stiles, &ctx->isplit, &ctx->srank, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries, &ctx->items,
                   &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1], &ctx->h_in[2],
                   &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass};
   for (DBuf* b : bufs) b->release();
@@ -1164,7 +1176,8 @@ asohf_status asohf_hip_get_stats(const asohf_ctx* cctx, asohf_stats* out) {
       double t4 = 0;
       HIPCHK(ms(0, 1, &ctx->stats.ms_bin), "timing");
       HIPCHK(ms(2, 3, &ctx->stats.ms_scatter), "timing");
-      HIPCHK(ms(4, 5, &ctx->stats.ms_deposit), "timing");
+      HIPCHK(ms(4, 8, &ctx->stats.ms_deposit), "timing");
+      HIPCHK(ms(8, 5, &ctx->stats.ms_split), "timing");
       HIPCHK(ms(3, 4, &t4), "timing");
       double t6 = 0;
       HIPCHK(ms(5, 6, &t6), "timing");

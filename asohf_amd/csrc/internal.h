@@ -142,8 +142,9 @@ struct DepArgs {
   const int4* entries;       // binned entries, grouped by tile
   const Item* items;         // work queue
   const int* nitems_tile;    // items per tile
-  const int* split_slot;     // scratch slot of a split tile, -1 otherwise
-  unsigned* done;            // finished items per split tile
+  const int* split_slot;     // first scratch slot of a split tile, -1 otherwise
+  const int* split_tiles;    // the split tiles, in tile order
+  const unsigned long long* n_split_tiles;   // their count (device)
   void* scratch;             // int64 (deterministic) or FP32 (atomic) tile accumulators
   Counters* C;               // n_items_eff (the queue length) and mmax_bits (the fixed-point
                              // scale) are read here, so no host round trip precedes the deposit

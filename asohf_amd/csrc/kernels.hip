@@ -404,7 +404,7 @@ __device__ __forceinline__ void tile_chunks(const unsigned long long* start, int
 }
 
 __global__ void plan_count(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
-                           unsigned* nitems_tile, unsigned* nsplit_tile) {
+                           unsigned* nitems_tile, unsigned* nsplit_tile, unsigned* is_split) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   long long s0, s1, s2;
@@ -413,11 +413,13 @@ __global__ void plan_count(const unsigned long long* start, int ntiles, int chun
   const int tot = max(cn + cw, 1);   // an empty tile still writes its zeros
   nitems_tile[t] = (unsigned)tot;
   nsplit_tile[t] = tot > 1 ? (unsigned)tot : 0u;
+  is_split[t] = tot > 1 ? 1u : 0u;
 }
 
 __global__ void plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
                            const unsigned long long* item_off, const unsigned long long* split_off,
-                           Item* items, int* split_slot, const Counters* C) {
+                           const unsigned long long* split_rank, Item* items, int* split_slot,
+                           int* split_tiles, long long* tile_mass, const Counters* C) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles || C->overflow) return;
   long long s0, s1, s2;
@@ -426,6 +428,10 @@ __global__ void plan_write(const unsigned long long* start, int ntiles, int chun
   const long long base = (long long)item_off[t];
   const int tot = max(cn + cw, 1);
   split_slot[t] = tot > 1 ? (int)split_off[t] : -1;
+  if (tot > 1) {
+    split_tiles[split_rank[t]] = t;
+    tile_mass[t] = 0;   // summed by split_reduce (zero bits: 0 as int64 and as FP64)
+  }
   if (cn + cw == 0) { items[base] = Item{t, 0, s0, s0}; return; }
   int k = 0;
   for (int c = 0; c < cn; ++c, ++k) {
@@ -561,7 +567,6 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
   constexpr int HX = TX + 4, HY = TY + 4, HZ = TZ + 4, H = HX * HY * HZ;
   __shared__ Acc acc[H];                 // the tile with a 2-cell halo
   __shared__ unsigned s_item;
-  __shared__ int s_last;
   __shared__ long long s_redi[16];
   __shared__ double s_redd[16];
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -689,32 +694,13 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
       __syncthreads();
     }
 
-    // ---- split tile: every item stores its partial interior; the last one
-    // to finish sums the partials in item order (fixed order: deterministic
-    // in both reductions) and writes the cells ----
+    // ---- split tile: every item stores its partial interior; split_reduce
+    // sums them in item order and writes the cells ----
     const int nit = D.nitems_tile[t];
-    bool writer = true;
-    if (nit > 1) {
-      Acc* part = reinterpret_cast<Acc*>(D.scratch) + (long long)D.split_slot[t] * T;
-      Acc* mine = part + (long long)(it.flags >> 8) * T;
+    const bool writer = nit == 1;
+    if (!writer) {
+      Acc* mine = reinterpret_cast<Acc*>(D.scratch) + ((long long)D.split_slot[t] + (it.flags >> 8)) * T;
       for (int c = tid; c < T; c += nthr) mine[c] = acc[at(c % TX, (c / TX) % TY, c / (TX * TY))];
-      __threadfence();
-      __syncthreads();
-      if (tid == 0) {
-        const unsigned prev = __hip_atomic_fetch_add(D.done + t, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = prev == (unsigned)(nit - 1);
-      }
-      __syncthreads();
-      writer = s_last;
-      if (writer) {
-        __threadfence();
-        for (int c = tid; c < T; c += nthr) {
-          Acc v = 0;
-          for (int k = 0; k < nit; ++k) v += glb_load(part + (long long)k * T + c);
-          acc[at(c % TX, (c / TX) % TY, c / (TX * TY))] = v;
-        }
-        __syncthreads();
-      }
     }
     if (writer) {
       float* out = (g == 0 ? D.u1 : D.u11) + G.off;
@@ -741,6 +727,59 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
       }
     }
     __syncthreads();
+  }
+}
+
+// Split tiles: the partial interiors of a tile's items summed in item order
+// (fixed order: deterministic in both reductions), normalised and written as
+// the deposit's writer does.  Work unit = (split tile, 256 consecutive cells),
+// so a heavy tile is spread over many workgroups; the exact int64 (FP64 in
+// atomic mode) tile mass is accumulated per unit.
+template <int RED, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(256) split_reduce(Plan P, DepArgs D) {
+  using Acc = typename AccT<RED>::T;
+  constexpr int T = TX * TY * TZ;
+  constexpr int kCells = 256;
+  constexpr int cpt = (T + kCells - 1) / kCells;
+  __shared__ long long s_redi[4];
+  __shared__ double s_redd[4];
+  if (D.C->overflow) return;
+  const double inv_scale = 1.0 / fixed_scale(D.C->mmax_bits);
+  const unsigned long long units = *D.n_split_tiles * cpt;
+  for (unsigned long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const int t = D.split_tiles[u / cpt];
+    const int c = (int)(u % cpt) * kCells + threadIdx.x;
+    const int g = P.tile_grid[t];
+    const GridDesc& G = P.grids[g];
+    const int local = t - G.tile0;
+    const int tx = local % G.nt[0], ty = (local / G.nt[0]) % G.nt[1], tz = local / (G.nt[0] * G.nt[1]);
+    const int i0 = tx * TX + 1, j0 = ty * TY + 1, k0c = tz * TZ + 1;
+    const int ex = min(TX, G.n[0] - i0 + 1), ey = min(TY, G.n[1] - j0 + 1), ez = min(TZ, G.n[2] - k0c + 1);
+    const int nit = D.nitems_tile[t];
+    const Acc* part = reinterpret_cast<const Acc*>(D.scratch) + (long long)D.split_slot[t] * T;
+    long long msi = 0;
+    double msd = 0.0;
+    const int a = c % TX, b = (c / TX) % TY, cz = c / (TX * TY);
+    if (c < T && a < ex && b < ey && cz < ez) {
+      Acc v = 0;
+#pragma unroll 4
+      for (int k = 0; k < nit; ++k) v += part[(long long)k * T + c];
+      if (RED == 0) msi = (long long)v;
+      else msd = (double)v;
+      float f = RED == 0 ? (float)((double)v * inv_scale) : (float)v;
+      f = f / G.den;
+      f = f / P.fdm;
+      float* out = (g == 0 ? D.u1 : D.u11) + G.off;
+      out[(long long)(i0 - 1 + a) + (long long)G.ld1 * (j0 - 1 + b) + (long long)G.ld2 * (k0c - 1 + cz)] = f;
+    }
+    if (RED == 0) {
+      msi = block_sum(msi, s_redi);
+      if (threadIdx.x == 0 && msi != 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(D.tile_mass_i + t), (unsigned long long)msi);
+    } else {
+      msd = block_sum(msd, s_redd);
+      if (threadIdx.x == 0 && msd != 0.0) atomicAdd(D.tile_mass_d + t, msd);
+    }
   }
 }
 
@@ -1107,17 +1146,19 @@ hipError_t launch_reduce_blocks(const BlockTables& BT, long long nblk, Counters*
 }
 
 hipError_t launch_plan_count(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
-                             unsigned* nitems_tile, unsigned* nsplit_tile, hipStream_t s) {
+                             unsigned* nitems_tile, unsigned* nsplit_tile, unsigned* is_split, hipStream_t s) {
   plan_count<<<grid_for(ntiles, 256, 1 << 30), 256, 0, s>>>(start, ntiles, chunk, chunk_wide, nitems_tile,
-                                                           nsplit_tile);
+                                                           nsplit_tile, is_split);
   return hipGetLastError();
 }
 
 hipError_t launch_plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
                              const unsigned long long* item_off, const unsigned long long* split_off,
-                             Item* items, int* split_slot, const Counters* C, hipStream_t s) {
+                             const unsigned long long* split_rank, Item* items, int* split_slot,
+                             int* split_tiles, long long* tile_mass, const Counters* C, hipStream_t s) {
   plan_write<<<grid_for(ntiles, 256, 1 << 30), 256, 0, s>>>(start, ntiles, chunk, chunk_wide, item_off,
-                                                           split_off, items, split_slot, C);
+                                                           split_off, split_rank, items, split_slot,
+                                                           split_tiles, tile_mass, C);
   return hipGetLastError();
 }
 
@@ -1147,6 +1188,22 @@ static hipError_t dep_shape(const Plan& P, const DepArgs& D, int grid, int block
 bool tile_shape_supported(const int t[3]) {
   return (t[0] == 8 && t[1] == 8 && t[2] == 8) || (t[0] == 16 && t[1] == 8 && t[2] == 8) ||
          (t[0] == 16 && t[1] == 16 && t[2] == 8) || (t[0] == 16 && t[1] == 16 && t[2] == 16);
+}
+
+template <int RED>
+static hipError_t split_shape(const Plan& P, const DepArgs& D, int grid, hipStream_t s) {
+  const int* t = P.tile;
+  if (t[0] == 8 && t[1] == 8 && t[2] == 8) split_reduce<RED, 8, 8, 8><<<grid, 256, 0, s>>>(P, D);
+  else if (t[0] == 16 && t[1] == 8 && t[2] == 8) split_reduce<RED, 16, 8, 8><<<grid, 256, 0, s>>>(P, D);
+  else if (t[0] == 16 && t[1] == 16 && t[2] == 8) split_reduce<RED, 16, 16, 8><<<grid, 256, 0, s>>>(P, D);
+  else if (t[0] == 16 && t[1] == 16 && t[2] == 16) split_reduce<RED, 16, 16, 16><<<grid, 256, 0, s>>>(P, D);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_split_reduce(const Plan& P, const DepArgs& D, int reduction, int cus, hipStream_t s) {
+  const int grid = (cus > 1 ? cus : 1) * 8;
+  return reduction == 0 ? split_shape<0>(P, D, grid, s) : split_shape<1>(P, D, grid, s);
 }
 
 hipError_t launch_deposit(const Plan& P, const DepArgs& D, int reduction, int grid, int block, hipStream_t s) {

@@ -20,15 +20,17 @@ hipError_t launch_scan(const unsigned* in, long long n, unsigned long long* bsum
                        unsigned long long* cursor, hipStream_t s);
 hipError_t launch_reduce_blocks(const BlockTables& BT, long long nblk, Counters* C, hipStream_t s);
 hipError_t launch_plan_count(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
-                             unsigned* nitems_tile, unsigned* nsplit_tile, hipStream_t s);
+                             unsigned* nitems_tile, unsigned* nsplit_tile, unsigned* is_split, hipStream_t s);
 hipError_t launch_plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
                              const unsigned long long* item_off, const unsigned long long* split_off,
-                             Item* items, int* split_slot, const Counters* C, hipStream_t s);
+                             const unsigned long long* split_rank, Item* items, int* split_slot,
+                             int* split_tiles, long long* tile_mass, const Counters* C, hipStream_t s);
 hipError_t launch_gate(const unsigned long long* tot_entries, const unsigned long long* tot_items,
                        const unsigned long long* tot_split, unsigned long long cap_entries,
                        unsigned long long cap_items, unsigned long long cap_split, Counters* C, hipStream_t s);
 bool tile_shape_supported(const int t[3]);
 hipError_t launch_deposit(const Plan& P, const DepArgs& D, int reduction, int grid, int block, hipStream_t s);
+hipError_t launch_split_reduce(const Plan& P, const DepArgs& D, int reduction, int cus, hipStream_t s);
 hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, double* grid_mass, hipStream_t s);
 hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
                           float* dx, float* dy, float* dz, int cus, hipStream_t s);

@@ -104,8 +104,8 @@ typedef struct asohf_stats {
   double ms_bin;            /* count + scan + item planning (before the host read) */
   double ms_scatter;        /* scatter of the entries                              */
   double ms_deposit;        /* the persistent deposit kernel alone                 */
-  double ms_sums;           /* item writing, scratch clearing and grid sums        */
-  float level_min[10], level_max[10]; /* per-level min/max of the fields (grids.f:835,987) */
+  double ms_sums;           /* item writing and grid sums                          */
+  double ms_split;          /* summation of split tiles' partials (split_reduce)   */
 } asohf_stats;
 
 /* Context. device < 0 selects the current device. */
