@@ -74,9 +74,9 @@ __global__ void fill_u32_kernel(unsigned* p, unsigned v, long long n) {
 // full is never inserted later (the table only fills), so every emission of
 // that key takes the direct global path consistently in all phases.
 constexpr int kBinThreads = 256;
-constexpr int kBinPer = 4;                       // particles per thread per block
+constexpr int kBinPer = 2;                       // particles per thread per block
 constexpr int kBinChunk = kBinThreads * kBinPer;
-constexpr int kBinCap = 2048;                    // LDS table slots (power of two)
+constexpr int kBinCap = 1024;                    // LDS table slots (power of two)
 constexpr int kBinProbe = 32;
 
 __device__ __forceinline__ unsigned key_hash(unsigned k) {
@@ -279,7 +279,7 @@ __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, u
           const int h = lds_find(keys, key);
           const unsigned long long slot =
               h >= 0 ? base[h] + atomicAdd(&vals[h], 1u) : atomicAdd(&cursor[key], 1ull);
-          if (slot < cap) entries[slot] = e;   // the count pass sized the buffer exactly
+          if (slot < cap) entries[slot] = e;   // the gate checked the buffer against the count pass
         },
         [&](int, int, int, int) {}, err);
   }
@@ -522,7 +522,10 @@ __device__ __forceinline__ void deposit27(typename AccT<RED>::T* base, float m, 
         for (int a = 0; a < 3; ++a)
           lds_add(reinterpret_cast<float*>(base) + (a + HX * (b + HY * c)), mxy[b][a] * wz[c]);
   } else {
-    int rest = 0;
+    // every term is >= 0 (masses >= 0, TSC and kernel weights >= 0) and
+    // below 2^31, so it converts to an unsigned word whose 64-bit form needs
+    // no sign extension
+    unsigned rest = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -530,11 +533,12 @@ __device__ __forceinline__ void deposit27(typename AccT<RED>::T* base, float m, 
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           if (a == 1 && b == 1 && c == 1) continue;
-          const int q = __float2int_rn(mxy[b][a] * wz[c]);
+          const unsigned q = __float2uint_rn(mxy[b][a] * wz[c]);
           rest += q;
-          lds_add(reinterpret_cast<long long*>(base) + (a + HX * (b + HY * c)), (long long)q);
+          lds_add(reinterpret_cast<long long*>(base) + (a + HX * (b + HY * c)), (long long)(unsigned long long)q);
         }
-    lds_add(reinterpret_cast<long long*>(base) + (1 + HX * (1 + HY * 1)), (long long)(__float2int_rn(ms) - rest));
+    lds_add(reinterpret_cast<long long*>(base) + (1 + HX * (1 + HY * 1)),
+            (long long)__float2uint_rn(ms) - (long long)rest);
   }
 }
 
@@ -600,57 +604,68 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
     __syncthreads();
 
     if (!(it.flags & 1)) {
-      // ---- narrow entries: one lane per entry, 27 cells; each lane walks
+      // ---- narrow entries: one lane per entry, 27 cells.  Each lane walks
       // its own contiguous stretch of the item (the 64 lanes of a wave hold
-      // particles from across the tile's list), two entries per pass with
-      // both loads in flight before the first is used ----
+      // particles from across the tile's list, so their stencils rarely
+      // collide), with the loads of the next two entries in flight while the
+      // current one is deposited ----
       const long long len = it.end - it.begin;
       const long long per = (len + nthr - 1) / nthr;
-      const long long lane_end = min(it.begin + (long long)(tid + 1) * per, it.end);
-      for (long long e0 = it.begin + (long long)tid * per; e0 < lane_end; e0 += 2) {
-        const long long e1 = e0 + 1;
-        const int4 v0 = D.entries[e0];
-        const int4 v1 = e1 < lane_end ? D.entries[e1] : make_int4(0, 0, 0, 0);
+      const long long lb = it.begin + (long long)tid * per;
+      const long long le = min(lb + per, it.end);
+      // Two entries per pass in two registers that are refilled right after
+      // use (no register copies, which would wait on the loads); the
+      // prefetch indices are clamped into the item instead of tested, so
+      // every pass issues the same loads.  An empty item reads nothing: no
+      // lane enters the loop.
+      auto narrow = [&](const int4 v) {
+        int ci[3];
+        float wx[3], wy[3], wz[3], m;
+        if (MODE == 2) {
+          const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
+          m = __int_as_float(v.w);
+          float* w[3] = {wx, wy, wz};
+          if (g == 0) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          if (u == 1 && e1 >= lane_end) break;
-          const int4 v = u == 0 ? v0 : v1;
-          int ci[3];
-          float wx[3], wy[3], wz[3], m;
-          if (MODE == 2) {
-            const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
-            m = __int_as_float(v.w);
-            float* w[3] = {wx, wy, wz};
-            if (g == 0) {
-#pragma unroll
-              for (int a = 0; a < 3; ++a) {
-                ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]);
-                tsc_weights_r(p[a], base_centre(P, a, ci[a] - 1, G.n[a]), base_centre(P, a, ci[a], G.n[a]),
-                              base_centre(P, a, ci[a] + 1, G.n[a]), G.inv_d[a], w[a]);
-              }
-            } else {
-#pragma unroll
-              for (int a = 0; a < 3; ++a) {
-                ci[a] = (int)index_quotient(p[a] - G.org[a], G.d[a], G.inv_d[a]);
-                tsc_weights_r(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
-                              minirad(G.pr[a], ci[a] + 1, G.d[a]), G.inv_d[a], w[a]);
-              }
+            for (int a = 0; a < 3; ++a) {
+              ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]);
+              tsc_weights_r(p[a], base_centre(P, a, ci[a] - 1, G.n[a]), base_centre(P, a, ci[a], G.n[a]),
+                            base_centre(P, a, ci[a] + 1, G.n[a]), G.inv_d[a], w[a]);
             }
           } else {
-            int irk;
-            unpack_table(v, ci[0], ci[1], ci[2], irk, m);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) { wx[k] = k0[k]; wy[k] = k0[k]; wz[k] = k0[k]; }
+            for (int a = 0; a < 3; ++a) {
+              ci[a] = (int)index_quotient(p[a] - G.org[a], G.d[a], G.inv_d[a]);
+              tsc_weights_r(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
+                            minirad(G.pr[a], ci[a] + 1, G.d[a]), G.inv_d[a], w[a]);
+            }
           }
-          // in [-1, ext] by construction of the binning; clamped so the array
-          // bounds never depend on that
-          const int lx = clampi(centre_local(ci[0], i0, ex, G.n[0], periodic), -1, TX);
-          const int ly = clampi(centre_local(ci[1], j0, ey, G.n[1], periodic), -1, TY);
-          const int lz = clampi(centre_local(ci[2], k0c, ez, G.n[2], periodic), -1, TZ);
-          // corner (-1,-1,-1) of the stencil; halo coordinates are local + 2
-          Acc* base = acc + ((lx + 1) + HX * ((ly + 1) + HY * (lz + 1)));
-          deposit27<RED, HX, HY>(base, m, wx, wy, wz, scale);
+        } else {
+          int irk;
+          unpack_table(v, ci[0], ci[1], ci[2], irk, m);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) { wx[k] = k0[k]; wy[k] = k0[k]; wz[k] = k0[k]; }
         }
+        // in [-1, ext] by construction of the binning; clamped so the array
+        // bounds never depend on that
+        const int lx = clampi(centre_local(ci[0], i0, ex, G.n[0], periodic), -1, TX);
+        const int ly = clampi(centre_local(ci[1], j0, ey, G.n[1], periodic), -1, TY);
+        const int lz = clampi(centre_local(ci[2], k0c, ez, G.n[2], periodic), -1, TZ);
+        // corner (-1,-1,-1) of the stencil; halo coordinates are local + 2
+        Acc* base = acc + ((lx + 1) + HX * ((ly + 1) + HY * (lz + 1)));
+        deposit27<RED, HX, HY>(base, m, wx, wy, wz, scale);
+      };
+      const long long last = it.end - 1;
+      int4 va = make_int4(0, 0, 0, 0), vb = va;
+      if (len > 0) {
+        va = D.entries[min(lb, last)];
+        vb = D.entries[min(lb + 1, last)];
+      }
+      for (long long e = lb; e < le; e += 2) {
+        narrow(va);
+        va = D.entries[min(e + 2, last)];
+        if (e + 1 < le) narrow(vb);
+        vb = D.entries[min(e + 3, last)];
       }
     } else if (MODE == 1) {
       // ---- wide table-mode entries: one wave per entry, lanes over cells ----
