@@ -604,20 +604,14 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
     __syncthreads();
 
     if (!(it.flags & 1)) {
-      // ---- narrow entries: one lane per entry, 27 cells.  Each lane walks
-      // its own contiguous stretch of the item (the 64 lanes of a wave hold
-      // particles from across the tile's list, so their stencils rarely
-      // collide), with the loads of the next two entries in flight while the
-      // current one is deposited ----
+      // ---- narrow entries: one lane per entry, 27 cells.  Lane j of the
+      // workgroup takes entries j, j + nthr, ... of the item, so a wave reads
+      // 64 consecutive entries (1 KB, every entry fetched once).  Two entries
+      // per pass, in two registers refilled right after use (register copies
+      // would wait on the loads); the prefetch indices are clamped into the
+      // item instead of tested, so every pass issues the same loads.  An
+      // empty item reads nothing: no lane enters the loop.
       const long long len = it.end - it.begin;
-      const long long per = (len + nthr - 1) / nthr;
-      const long long lb = it.begin + (long long)tid * per;
-      const long long le = min(lb + per, it.end);
-      // Two entries per pass in two registers that are refilled right after
-      // use (no register copies, which would wait on the loads); the
-      // prefetch indices are clamped into the item instead of tested, so
-      // every pass issues the same loads.  An empty item reads nothing: no
-      // lane enters the loop.
       auto narrow = [&](const int4 v) {
         int ci[3];
         float wx[3], wy[3], wz[3], m;
@@ -657,15 +651,16 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
       };
       const long long last = it.end - 1;
       int4 va = make_int4(0, 0, 0, 0), vb = va;
+      const long long e0 = it.begin + tid;
       if (len > 0) {
-        va = D.entries[min(lb, last)];
-        vb = D.entries[min(lb + 1, last)];
+        va = D.entries[min(e0, last)];
+        vb = D.entries[min(e0 + nthr, last)];
       }
-      for (long long e = lb; e < le; e += 2) {
+      for (long long e = e0; e < it.end; e += 2 * nthr) {
         narrow(va);
-        va = D.entries[min(e + 2, last)];
-        if (e + 1 < le) narrow(vb);
-        vb = D.entries[min(e + 3, last)];
+        va = D.entries[min(e + 2 * nthr, last)];
+        if (e + nthr < it.end) narrow(vb);
+        vb = D.entries[min(e + 3 * nthr, last)];
       }
     } else if (MODE == 1) {
       // ---- wide table-mode entries: one wave per entry, lanes over cells ----
