@@ -818,6 +818,17 @@ __global__ void __launch_bounds__(256) grid_sums(Plan P, DepArgs D, double* grid
 // adjoint of the TSC mode: one thread per particle, gathers the upstream
 // gradient at every stencil cell of every grid the particle belongs to
 // ---------------------------------------------------------------------------
+// three consecutive cells of a row in one load (dword alignment suffices)
+struct __attribute__((aligned(4))) Row3 {
+  float v[3];
+};
+__device__ __forceinline__ void load_row3(const float* p, float out[3]) {
+  const Row3 r = *reinterpret_cast<const Row3*>(p);
+  out[0] = r.v[0];
+  out[1] = r.v[1];
+  out[2] = r.v[2];
+}
+
 __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const float* g1, const float* g11,
                                                       float* dm, float* dxo, float* dyo, float* dzo) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
@@ -839,29 +850,41 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
         tsc_dweights(p[a], c0, c1, c2, G.d[a], dw[a]);
       }
       const float sc = 1.0f / (G.den * P.fdm);
+      // the three x cells of a row are contiguous unless the row wraps
+      const bool xin = ci[0] >= 2 && ci[0] <= G.n[0] - 1;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const long long kz = wrap1(ci[2] - 1 + c, G.n[2]) - 1;
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
           const long long jy = wrap1(ci[1] - 1 + b, G.n[1]) - 1;
+          const float* row = g1 + G.off + G.ld1 * jy + G.ld2 * kz;
+          float gv[3];
+          if (xin) {
+            load_row3(row + (ci[0] - 2), gv);
+          } else {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) gv[a] = row[wrap1(ci[0] - 1 + a, G.n[0]) - 1];
+          }
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            const long long ix = wrap1(ci[0] - 1 + a, G.n[0]) - 1;
-            const float gv = g1[G.off + ix + G.ld1 * jy + G.ld2 * kz] * sc;
-            am += gv * w[0][a] * w[1][b] * w[2][c];
-            ax += gv * dw[0][a] * w[1][b] * w[2][c];
-            ay += gv * w[0][a] * dw[1][b] * w[2][c];
-            az += gv * w[0][a] * w[1][b] * dw[2][c];
+            const float v = gv[a] * sc;
+            am += v * w[0][a] * w[1][b] * w[2][c];
+            ax += v * dw[0][a] * w[1][b] * w[2][c];
+            ay += v * w[0][a] * dw[1][b] * w[2][c];
+            az += v * w[0][a] * w[1][b] * dw[2][c];
           }
         }
       }
     }
+    // the descent stops where the forward enumeration stops (nested boxes)
     for (int l = 1; l <= P.nl; ++l) {
       const int2 se = bucket_lookup(P, l, p[0], p[1], p[2]);
+      bool any = false;
       for (int q = 0; q < se.y; ++q) {
         const BoxRec r = P.hrec[se.x + q];
         if (!in_box(r, p[0], p[1], p[2])) continue;
+        any = true;
         const int g = r.g;
         const GridDesc& G = P.grids[g];
         int ci[3];
@@ -876,6 +899,7 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
         }
         const float sc = 1.0f / (G.den * P.fdm);
         const float* gp = g11 + G.off;
+        const bool xin = ci[0] >= 2 && ci[0] <= G.n[0] - 1;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const int kz = ci[2] - 1 + c;
@@ -884,19 +908,29 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
           for (int b = 0; b < 3; ++b) {
             const int jy = ci[1] - 1 + b;
             if (jy < 1 || jy > G.n[1]) continue;
+            const float* row = gp + (long long)G.ld1 * (jy - 1) + (long long)G.ld2 * (kz - 1);
+            float gv[3];
+            if (xin) {
+              load_row3(row + (ci[0] - 2), gv);
+            } else {
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                const int ix = ci[0] - 1 + a;
+                gv[a] = (ix >= 1 && ix <= G.n[0]) ? row[ix - 1] : 0.f;
+              }
+            }
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-              const int ix = ci[0] - 1 + a;
-              if (ix < 1 || ix > G.n[0]) continue;
-              const float gv = gp[(long long)(ix - 1) + (long long)G.ld1 * (jy - 1) + (long long)G.ld2 * (kz - 1)] * sc;
-              am += gv * w[0][a] * w[1][b] * w[2][c];
-              ax += gv * dw[0][a] * w[1][b] * w[2][c];
-              ay += gv * w[0][a] * dw[1][b] * w[2][c];
-              az += gv * w[0][a] * w[1][b] * dw[2][c];
+              const float v = gv[a] * sc;
+              am += v * w[0][a] * w[1][b] * w[2][c];
+              ax += v * dw[0][a] * w[1][b] * w[2][c];
+              ay += v * w[0][a] * dw[1][b] * w[2][c];
+              az += v * w[0][a] * w[1][b] * dw[2][c];
             }
           }
         }
       }
+      if (!any && P.nested) break;
     }
     dm[i] = am;
     dxo[i] = m * ax;

@@ -277,7 +277,7 @@ def cpu_baseline(args, snap, dev_xyzm, u1_gpu, n, P):
 
     from oracle import oracle as O
 
-    cores = args.cpu_threads or len(os.sched_getaffinity(0))
+    cores = args.cpu_threads or min(len(os.sched_getaffinity(0)), 16)
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
     mode = "B" if args.mode == "tsc" else "A"
     name = "INTERPOLATE_DENSITY_TSC" if mode == "B" else "DENSITY"
