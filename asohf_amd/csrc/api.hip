@@ -177,6 +177,7 @@ struct asohf_ctx {
   hipGraphExec_t gexec = nullptr;   // the captured batched build and the arguments it holds
   PipeArgs gkey;
   DBuf h_in[4], h_out[2], h_mass;    // device copies for the host entry point
+  DBuf ord[4], ord_tmp;              // adjoint gather order: keys and indices (in, out), sort scratch
   Counters* hcount = nullptr;          // pinned
   unsigned long long* hscalar = nullptr;
   hipEvent_t ev[9] = {};   // 0 start, 1 count pass, 2 scatter start, 3 scatter end, 4 deposit start,
@@ -187,6 +188,9 @@ struct asohf_ctx {
   bool stats_phases = false;
   std::map<std::pair<int, int>, std::vector<float>> ktab_cache;
 };
+
+// particles from which the adjoint gathers in base-cell order
+constexpr long long kAdjointSortMin = 1ll << 20;
 
 static asohf_status fail(asohf_ctx* c, asohf_status st, const std::string& msg) {
   if (c) c->err = msg;
@@ -944,7 +948,8 @@ void asohf_hip_destroy(asohf_ctx* ctx) {
                   &ctx->item_off, &ctx->split_off, &ctx->plan_blob, &ctx->cnt, &ctx->start, &ctx->cursor, &ctx->bsum, &ctx->counters,
                   &ctx->stiles, &ctx->isplit, &ctx->srank, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries, &ctx->items,
                   &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1], &ctx->h_in[2],
-                  &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass};
+                  &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass, &ctx->ord[0], &ctx->ord[1],
+                  &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp};
   for (DBuf* b : bufs) b->release();
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1048,7 +1053,30 @@ asohf_status asohf_hip_density_adjoint(asohf_ctx* ctx, const asohf_geom* geom, i
   if (st != ASOHF_OK) return st;
   hipStream_t s = (hipStream_t)o.stream;   // NULL: the default stream
   HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
-  if (n > 0) HIPCHK(launch_adjoint(ctx->plan.dev, A, g1, g11, dm, dxp, dyp, dzp, ctx->cus, s), "adjoint kernel");
+  // Large inputs are gathered in base-cell order (a radix sort of (cell,
+  // index) pairs): neighbouring lanes then read the same gradient rows.
+  const unsigned* order = nullptr;
+  if (n >= kAdjointSortMin && n < (1ll << 32)) {
+    const long long nc = (long long)geom->nx * geom->ny * geom->nz;
+    int bits = 1;
+    while (bits < 32 && (1ll << bits) < nc) ++bits;
+    const size_t b4 = sizeof(unsigned) * (size_t)n;
+    HIPCHK(ctx->ord[0].ensure(b4), "allocating the gather order");
+    HIPCHK(ctx->ord[1].ensure(b4), "allocating the gather order");
+    HIPCHK(ctx->ord[2].ensure(b4), "allocating the gather order");
+    HIPCHK(ctx->ord[3].ensure(b4), "allocating the gather order");
+    size_t tb = 0;
+    HIPCHK(sort_cell_pairs(nullptr, tb, nullptr, nullptr, nullptr, nullptr, (unsigned long long)n, bits, s),
+           "sizing the sort");
+    HIPCHK(ctx->ord_tmp.ensure(std::max<size_t>(tb, 1)), "allocating the sort");
+    HIPCHK(launch_cell_keys(ctx->plan.dev, A, ctx->ord[0].as<unsigned>(), ctx->ord[1].as<unsigned>(), ctx->cus, s),
+           "cell keys");
+    HIPCHK(sort_cell_pairs(ctx->ord_tmp.p, tb, ctx->ord[0].as<unsigned>(), ctx->ord[2].as<unsigned>(),
+                           ctx->ord[1].as<unsigned>(), ctx->ord[3].as<unsigned>(), (unsigned long long)n, bits, s),
+           "sorting by cell");
+    order = ctx->ord[3].as<unsigned>();
+  }
+  if (n > 0) HIPCHK(launch_adjoint(ctx->plan.dev, A, g1, g11, dm, dxp, dyp, dzp, order, ctx->cus, s), "adjoint kernel");
   HIPCHK(hipEventRecord(ctx->ev[6], s), "event");
   ctx->stats = asohf_stats{};
   ctx->stats.n_particles = n;

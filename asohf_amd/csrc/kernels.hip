@@ -829,10 +829,30 @@ __device__ __forceinline__ void load_row3(const float* p, float out[3]) {
   out[2] = r.v[2];
 }
 
-__global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const float* g1, const float* g11,
-                                                      float* dm, float* dxo, float* dyo, float* dzo) {
+// Base-cell key (x fastest) of every particle, for the gather order of the
+// adjoint: lanes of a wave that take particles of one cell read the same
+// rows of the upstream gradient.
+__global__ void __launch_bounds__(256) cell_keys(Plan P, Parts A, unsigned* key, unsigned* idx) {
+  const GridDesc& G = P.grids[0];
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
        i += (long long)gridDim.x * blockDim.x) {
+    const float p[3] = {A.x[i], A.y[i], A.z[i]};
+    int c[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) c[a] = clampi(tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]), 1, G.n[a]) - 1;
+    key[i] = (unsigned)c[0] + (unsigned)G.n[0] * ((unsigned)c[1] + (unsigned)G.n[1] * (unsigned)c[2]);
+    idx[i] = (unsigned)i;
+  }
+}
+
+// One thread per particle, in the given order (NULL: index order).  The
+// result of a particle does not depend on the order.
+__global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const float* g1, const float* g11,
+                                                      float* dm, float* dxo, float* dyo, float* dzo,
+                                                      const unsigned* order) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < A.n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long i = order ? (long long)order[t] : t;
     const float p[3] = {A.x[i], A.y[i], A.z[i]};
     const float m = A.m[i];
     float am = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
@@ -1262,9 +1282,14 @@ hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, doub
   return hipGetLastError();
 }
 
+hipError_t launch_cell_keys(const Plan& P, const Parts& A, unsigned* key, unsigned* idx, int cus, hipStream_t s) {
+  cell_keys<<<grid_for(A.n, 256, cus * 16), 256, 0, s>>>(P, A, key, idx);
+  return hipGetLastError();
+}
+
 hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
-                          float* dx, float* dy, float* dz, int cus, hipStream_t s) {
-  adjoint_gather<<<grid_for(A.n, 256, cus * 16), 256, 0, s>>>(P, A, g1, g11, dm, dx, dy, dz);
+                          float* dx, float* dy, float* dz, const unsigned* order, int cus, hipStream_t s) {
+  adjoint_gather<<<grid_for(A.n, 256, cus * 16), 256, 0, s>>>(P, A, g1, g11, dm, dx, dy, dz, order);
   return hipGetLastError();
 }
 

@@ -33,7 +33,11 @@ hipError_t launch_deposit(const Plan& P, const DepArgs& D, int reduction, int gr
 hipError_t launch_split_reduce(const Plan& P, const DepArgs& D, int reduction, int cus, hipStream_t s);
 hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, double* grid_mass, hipStream_t s);
 hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
-                          float* dx, float* dy, float* dz, int cus, hipStream_t s);
+                          float* dx, float* dy, float* dz, const unsigned* order, int cus, hipStream_t s);
+hipError_t launch_cell_keys(const Plan& P, const Parts& A, unsigned* key, unsigned* idx, int cus, hipStream_t s);
+hipError_t sort_cell_pairs(void* temp, size_t& temp_bytes, const unsigned* key_in, unsigned* key_out,
+                           const unsigned* idx_in, unsigned* idx_out, unsigned long long n, int end_bit,
+                           hipStream_t s);
 hipError_t launch_plain_zero(const Plan& P, float* u1, float* u11, hipStream_t s);
 hipError_t launch_plain_base(const Plan& P, const Parts& A, float* u1, int cus, hipStream_t s);
 hipError_t launch_plain_flags(const Plan& P, const Parts& A, int l, unsigned* flag, int cus, hipStream_t s);
