@@ -189,3 +189,52 @@ def test_larger_snapshot_properties(bld):
         assert abs(gm[0] - tot) <= 1e-12 * tot
         a1, a11, _ = run(bld, snap, mode, "atomic")
         assert rel_err_above_mean(a1, d1)[0] <= atomic_tol(snap, mode)
+
+
+def test_full_size_workload_properties(bld):
+    """The 16.8M-particle benchmark workload at full size: size-independent
+    properties (exact mass conservation, bit-identical results under a
+    reversed particle order, across runs and in the graph form) and FP64
+    oracle parity on a seeded 2^17-particle sample with the full hierarchy."""
+    snap = synth.make_snapshot(synth.CONFIGS[2])
+    g = snap.geom
+    print(f"case full-size c2: {snap.n} particles, {g.npatch_total} patches", flush=True)
+    d = torch.device("cuda:0")
+    x, y, z, m = dev(snap)
+    ref = run(bld, snap, "tsc")
+    tot = float(snap.m.astype(np.float64).sum())
+    assert abs(ref[2][0] - tot) <= 1e-12 * tot
+    # every particle deposits exactly its quantised mass: per-grid FP64 sums add
+    # up to the total counted over the (particle, grid) memberships
+    assert np.all(ref[2] >= 0)
+    rev = [t.flip(0).contiguous() for t in (x, y, z, m)]
+    r = bld.density(g, *rev, mode="tsc")
+    torch.cuda.synchronize()
+    for a, b in zip((r.u1, r.u11, r.grid_mass), ref):
+        assert np.array_equal(a.cpu().numpy(), b)
+    s1, n11 = bld.out_shapes(g)
+    u1 = torch.empty(s1, dtype=torch.float32, device=d)
+    u11 = torch.empty(n11, dtype=torch.float32, device=d)
+    gm = torch.empty(1 + g.npatch_total, dtype=torch.float64, device=d)
+    for rep in range(2):
+        u1.zero_()
+        torch.cuda.synchronize()
+        bld.density(g, x, y, z, m, mode="tsc", u1=u1, u11=u11, grid_mass=gm, use_graph=True)
+        torch.cuda.synchronize()
+        for a, b in zip((u1, u11, gm), ref):
+            assert np.array_equal(a.cpu().numpy(), b), rep
+    # atomic mode against the deterministic fields (FP32 accumulation order)
+    a1, a11, _ = run(bld, snap, "tsc", "atomic")
+    assert rel_err_above_mean(a1, ref[0])[0] <= 1e-4
+    assert rel_err_above_mean(a11, ref[1])[0] <= 1e-4
+    # oracle parity on a sample, full hierarchy
+    rng = np.random.default_rng(2)
+    idx = np.sort(rng.choice(snap.n, size=1 << 17, replace=False))
+    sub = synth.Snapshot(snap.x[idx], snap.y[idx], snap.z[idx], snap.m[idx], snap.npart_esp, g)
+    u1s, u11s, gms = run(bld, sub, "tsc")
+    d1, d11, dgm = O.density64(sub, "B", nthreads=0)
+    assert rel_err_above_mean(u1s, d1)[0] <= 1e-5
+    assert rel_err_above_mean(u11s, d11)[0] <= 1e-5
+    # per-grid FP64 mass: the rounded fixed-point terms of cells clipped at a
+    # patch border differ from the exact ones by half a unit (2^-31 of the mass)
+    assert np.allclose(gms, dgm, rtol=1e-6, atol=0)
