@@ -443,7 +443,9 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   while (cap < 2 * ukeys.size() + 2) cap <<= 1;
   H.hslots.assign(cap, HashSlot{kHashEmpty, 0, 0});
   for (size_t i = 0; i < ukeys.size(); ++i) {
-    unsigned s = (unsigned)hash_mix(ukeys[i]) & (unsigned)(cap - 1);
+    const uint64_t k = ukeys[i];
+    unsigned s = bucket_hash((int)(k >> 60), (int)((k >> 40) & 0xfffff), (int)((k >> 20) & 0xfffff),
+                             (int)(k & 0xfffff)) & (unsigned)(cap - 1);
     while (H.hslots[s].key != kHashEmpty) s = (s + 1) & (unsigned)(cap - 1);
     H.hslots[s] = HashSlot{ukeys[i], uvals[i].x, uvals[i].y};
   }

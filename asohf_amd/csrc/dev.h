@@ -70,12 +70,28 @@ __device__ __forceinline__ int tsc_base_index(float xp, float xl, float d, float
   return clampi((int)t + 1, 1, n);
 }
 
-__host__ __device__ __forceinline__ uint64_t hash_mix(uint64_t k) {
-  k ^= k >> 33; k *= 0xff51afd7ed558ccdull; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ull; k ^= k >> 33;
-  return k;
-}
 __host__ __device__ __forceinline__ uint64_t bucket_key(int level, int bx, int by, int bz) {
   return ((uint64_t)level << 60) | ((uint64_t)bx << 40) | ((uint64_t)by << 20) | (uint64_t)bz;
+}
+
+// low 32 bits of the product of the low 24 bits of a and b (one full-rate
+// v_mul_u32_u24 on the device)
+__host__ __device__ __forceinline__ unsigned mul24(unsigned a, unsigned b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __umul24(a, b);
+#else
+  return (a & 0xffffffu) * (b & 0xffffffu);
+#endif
+}
+
+// slot hash of a bucket (coordinates < 2^14, level < 16), 32-bit arithmetic
+__host__ __device__ __forceinline__ unsigned bucket_hash(int level, int bx, int by, int bz) {
+  unsigned h = mul24((unsigned)bx, 0x9E3779u) ^ mul24((unsigned)by, 0x85EBCBu) ^
+               mul24((unsigned)bz, 0xC2B2AFu) ^ mul24((unsigned)level + 1u, 0x27D4EBu);
+  h ^= h >> 15;
+  h = mul24(h, 0x2C1B3Du) ^ (h >> 17);
+  h ^= h >> 13;
+  return h;
 }
 
 // candidate patches of level l whose (inflated) membership box may contain (x,y,z)
@@ -85,7 +101,7 @@ __device__ __forceinline__ int2 bucket_lookup(const Plan& P, int l, float x, flo
   int by = clampi((int)floorf(fminf(fmaxf((y - h.x0) * h.inv_bs, -1.f), 1.0e6f)), 0, h.nb - 1);
   int bz = clampi((int)floorf(fminf(fmaxf((z - h.x0) * h.inv_bs, -1.f), 1.0e6f)), 0, h.nb - 1);
   const uint64_t key = bucket_key(l, bx, by, bz);
-  unsigned s = (unsigned)hash_mix(key) & P.hmask;
+  unsigned s = bucket_hash(l, bx, by, bz) & P.hmask;
   // the table is at most half full, so an empty slot ends every probe
   for (;;) {
     const HashSlot h = P.hslots[s];
