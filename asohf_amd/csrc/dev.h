@@ -160,17 +160,24 @@ template <int MODE, class Emit, class Member>
 __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, float x, float y,
                                                    float z, float m, int species, Emit&& emit,
                                                    Member&& member, unsigned& err) {
-  const GridDesc& B = P.grids[0];
+  // Grid fields are copied into registers before the emission loops: the
+  // callbacks write memory (atomics, entries), after which the compiler would
+  // otherwise reload every field from the descriptor.
+  const GridDesc& Bref = P.grids[0];
+  const int bn[3] = {Bref.n[0], Bref.n[1], Bref.n[2]};
+  const float bd[3] = {Bref.d[0], Bref.d[1], Bref.d[2]};
+  const float bid[3] = {Bref.inv_d[0], Bref.inv_d[1], Bref.inv_d[2]};
+  const int btile0 = Bref.tile0, bnt0 = Bref.nt[0], bnt1 = Bref.nt[1];
   // ---------------- base grid ----------------
   int ix, jy, kz;
   if (MODE == 2) {
-    ix = tsc_base_index(x, P.xl, B.d[0], B.inv_d[0], B.n[0]);
-    jy = tsc_base_index(y, P.xl, B.d[1], B.inv_d[1], B.n[1]);
-    kz = tsc_base_index(z, P.xl, B.d[2], B.inv_d[2], B.n[2]);
+    ix = tsc_base_index(x, P.xl, bd[0], bid[0], bn[0]);
+    jy = tsc_base_index(y, P.xl, bd[1], bid[1], bn[1]);
+    kz = tsc_base_index(z, P.xl, bd[2], bid[2], bn[2]);
   } else {
-    const float tx = index_quotient(x - P.xl, B.d[0], B.inv_d[0]);
-    const float ty = index_quotient(y - P.xl, B.d[1], B.inv_d[1]);
-    const float tz = index_quotient(z - P.xl, B.d[2], B.inv_d[2]);
+    const float tx = index_quotient(x - P.xl, bd[0], bid[0]);
+    const float ty = index_quotient(y - P.xl, bd[1], bid[1]);
+    const float tz = index_quotient(z - P.xl, bd[2], bid[2]);
     // INT() is defined only on a bounded range; beyond it the reference would
     // index outside U1 anyway
     if (!(tx > -1.0e9f && tx < 1.0e9f && ty > -1.0e9f && ty < 1.0e9f && tz > -1.0e9f && tz < 1.0e9f)) {
@@ -179,22 +186,22 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
     }
     ix = (int)tx + 1; jy = (int)ty + 1; kz = (int)tz + 1;
     // the single wrap must land in 1..N for all three offsets
-    if (ix < 2 - B.n[0] || ix > 2 * B.n[0] - 1 || jy < 2 - B.n[1] || jy > 2 * B.n[1] - 1 ||
-        kz < 2 - B.n[2] || kz > 2 * B.n[2] - 1) { err |= kErrWrap; return; }
+    if (ix < 2 - bn[0] || ix > 2 * bn[0] - 1 || jy < 2 - bn[1] || jy > 2 * bn[1] - 1 ||
+        kz < 2 - bn[2] || kz > 2 * bn[2] - 1) { err |= kErrWrap; return; }
   }
   member(0, ix, jy, kz);
   {
     int tx[3], ty[3], tz[3];
-    const int nx_ = base_axis_tiles(ix, B.n[0], P.tshift[0], tx);
-    const int ny_ = base_axis_tiles(jy, B.n[1], P.tshift[1], ty);
-    const int nz_ = base_axis_tiles(kz, B.n[2], P.tshift[2], tz);
+    const int nx_ = base_axis_tiles(ix, bn[0], P.tshift[0], tx);
+    const int ny_ = base_axis_tiles(jy, bn[1], P.tshift[1], ty);
+    const int nz_ = base_axis_tiles(kz, bn[2], P.tshift[2], tz);
     int4 e;
     if (MODE == 2) e = make_int4(__float_as_int(x), __float_as_int(y), __float_as_int(z), __float_as_int(m));
     else e = pack_table(ix, jy, kz, 0, m);
     for (int c = 0; c < nz_; ++c)
       for (int b = 0; b < ny_; ++b)
         for (int a = 0; a < nx_; ++a)
-          emit(0, B.tile0 + (tz[c] * B.nt[1] + ty[b]) * B.nt[0] + tx[a], 0, e);
+          emit(0, btile0 + (tz[c] * bnt1 + ty[b]) * bnt0 + tx[a], 0, e);
   }
   // ---------------- patches, level by level ----------------
   // When every patch's membership box lies inside a box of the level above
@@ -207,26 +214,32 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
       const BoxRec r = P.hrec[se.x + q];
       if (!in_box(r, x, y, z)) continue;
       const int g = r.g;
-      const GridDesc& G = P.grids[g];
+      const GridDesc& Gref = P.grids[g];
+      const float gorg[3] = {Gref.org[0], Gref.org[1], Gref.org[2]};
+      const float gd[3] = {Gref.d[0], Gref.d[1], Gref.d[2]};
+      const float gid[3] = {Gref.inv_d[0], Gref.inv_d[1], Gref.inv_d[2]};
+      const int gn[3] = {Gref.n[0], Gref.n[1], Gref.n[2]};
+      const int gtile0 = Gref.tile0, gnt0 = Gref.nt[0], gnt1 = Gref.nt[1];
+      const int glevel = Gref.level;
       any = true;
       int lo[3], hi[3], ci[3];
       int irk = 0;
       if (MODE == 2) {
-        ci[0] = (int)index_quotient(x - G.org[0], G.d[0], G.inv_d[0]);
-        ci[1] = (int)index_quotient(y - G.org[1], G.d[1], G.inv_d[1]);
-        ci[2] = (int)index_quotient(z - G.org[2], G.d[2], G.inv_d[2]);
+        ci[0] = (int)index_quotient(x - gorg[0], gd[0], gid[0]);
+        ci[1] = (int)index_quotient(y - gorg[1], gd[1], gid[1]);
+        ci[2] = (int)index_quotient(z - gorg[2], gd[2], gid[2]);
         member(g, ci[0], ci[1], ci[2]);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { lo[a] = max(ci[a] - 1, 1); hi[a] = min(ci[a] + 1, G.n[a]); }
+        for (int a = 0; a < 3; ++a) { lo[a] = max(ci[a] - 1, 1); hi[a] = min(ci[a] + 1, gn[a]); }
       } else {
-        ci[0] = (int)floorf(index_quotient(x - G.org[0], G.d[0], G.inv_d[0])) + 1;
-        ci[1] = (int)floorf(index_quotient(y - G.org[1], G.d[1], G.inv_d[1])) + 1;
-        ci[2] = (int)floorf(index_quotient(z - G.org[2], G.d[2], G.inv_d[2])) + 1;
+        ci[0] = (int)floorf(index_quotient(x - gorg[0], gd[0], gid[0])) + 1;
+        ci[1] = (int)floorf(index_quotient(y - gorg[1], gd[1], gid[1])) + 1;
+        ci[2] = (int)floorf(index_quotient(z - gorg[2], gd[2], gid[2])) + 1;
         member(g, ci[0], ci[1], ci[2]);
-        irk = max(G.level - species, 0);
+        irk = max(glevel - species, 0);
         const int nk = 1 << irk;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { lo[a] = max(ci[a] - nk, 1); hi[a] = min(ci[a] + nk, G.n[a]); }
+        for (int a = 0; a < 3; ++a) { lo[a] = max(ci[a] - nk, 1); hi[a] = min(ci[a] + nk, gn[a]); }
       }
       if (hi[0] < lo[0] || hi[1] < lo[1] || hi[2] < lo[2]) continue;   // cloud misses the patch
       int4 e;
@@ -239,7 +252,7 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
       for (int c = t0z; c <= t1z; ++c)
         for (int b = t0y; b <= t1y; ++b)
           for (int a = t0x; a <= t1x; ++a)
-            emit(g, G.tile0 + (c * G.nt[1] + b) * G.nt[0] + a, wide, e);
+            emit(g, gtile0 + (c * gnt1 + b) * gnt0 + a, wide, e);
     }
     if (!any && P.nested) break;
   }

@@ -79,16 +79,26 @@ constexpr int kBinChunk = kBinThreads * kBinPer;
 constexpr int kBinCap = 1024;                    // LDS table slots (power of two)
 constexpr int kBinProbe = 32;
 
+// tile keys are < 2^24 (2 * tiles); 24-bit multiplies are full rate
 __device__ __forceinline__ unsigned key_hash(unsigned k) {
-  k ^= k >> 16; k *= 0x7feb352du; k ^= k >> 15; k *= 0x846ca68bu; k ^= k >> 16;
+  k ^= k >> 13;
+  k = __umul24(k, 0x9E3779u) ^ (k >> 11);
+  k ^= k >> 7;
   return k & (kBinCap - 1);
 }
-// slot of key k, inserting it when absent; -1 when its probe path is full
+// slot of key k, inserting it when absent; -1 when its probe path is full.
+// A plain read first: most emissions find their key already present, and a
+// read costs less than a compare-and-swap.
 __device__ __forceinline__ int lds_insert(int* keys, int k) {
   unsigned h = key_hash((unsigned)k);
+#pragma unroll 1
   for (int p = 0; p < kBinProbe; ++p) {
-    const int old = atomicCAS(&keys[h], -1, k);
-    if (old == -1 || old == k) return (int)h;
+    const int cur = keys[h];
+    if (cur == k) return (int)h;
+    if (cur == -1) {
+      const int old = atomicCAS(&keys[h], -1, k);
+      if (old == -1 || old == k) return (int)h;
+    }
     h = (h + 1) & (kBinCap - 1);
   }
   return -1;
