@@ -172,7 +172,9 @@ struct asohf_ctx {
   DBuf plan_blob;
   DBuf cnt, start, cursor, bsum, counters, stiles, isplit, srank, nitems, split, tmass, entries, items, scratch;
   DBuf flags, idx, pool, pool_off, pool_nk, blk_pairs, blk_mmax, nsplit, item_off, split_off;
+  DBuf elist, eoff, ecount;
   long long pool_want = 0;   // pairs the last count pass needed
+  long long epool_want = 0;  // emission-list words the last count pass needed
   bool sized = false;        // the entry/item/scratch buffers were sized by an exact call
   hipGraphExec_t gexec = nullptr;   // the captured batched build and the arguments it holds
   PipeArgs gkey;
@@ -638,6 +640,18 @@ static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Part
   HIPCHK(ctx->pool_nk.ensure(sizeof(int) * std::max<long long>(nblk, 1)), "allocating block tables");
   HIPCHK(ctx->blk_pairs.ensure(sizeof(unsigned long long) * std::max<long long>(nblk, 1)), "allocating block sums");
   HIPCHK(ctx->blk_mmax.ensure(sizeof(unsigned) * std::max<long long>(nblk, 1)), "allocating block sums");
+  // emission lists (TSC): about four per particle until a call has measured them
+  a.BT.use_elist = (o.mode == ASOHF_MODE_TSC && 2ll * ntiles < (1ll << 23)) ? 1 : 0;
+  if (a.BT.use_elist) {
+    const long long ecap = std::max<long long>(ctx->epool_want + ctx->epool_want / 8, 4 * A.n + 4096);
+    HIPCHK(ctx->elist.ensure(sizeof(unsigned) * ecap), "allocating emission lists");
+    HIPCHK(ctx->eoff.ensure(sizeof(long long) * std::max<long long>(nblk, 1)), "allocating emission lists");
+    HIPCHK(ctx->ecount.ensure(sizeof(int) * std::max<long long>(nblk, 1)), "allocating emission lists");
+    a.BT.elist = ctx->elist.as<unsigned>();
+    a.BT.ecap = (long long)(ctx->elist.cap / sizeof(unsigned));
+    a.BT.eoff = ctx->eoff.as<long long>();
+    a.BT.ecount = ctx->ecount.as<int>();
+  }
   a.BT.pool = ctx->pool.as<int2>();
   a.BT.cap = (long long)(ctx->pool.cap / sizeof(int2));
   a.BT.off = ctx->pool_off.as<long long>();
@@ -739,6 +753,9 @@ static void note_pool(asohf_ctx* ctx, const Counters& hc) {
   unsigned long long sub_max = 0;
   for (int k = 0; k < kPoolSubs; ++k) sub_max = std::max(sub_max, hc.pool_fill[k * (kLine / 2)]);
   ctx->pool_want = (long long)(sub_max * kPoolSubs);
+  unsigned long long esub = 0;
+  for (int k = 0; k < kPoolSubs; ++k) esub = std::max(esub, hc.epool_fill[k * (kLine / 2)]);
+  ctx->epool_want = std::max(ctx->epool_want, (long long)(esub * kPoolSubs));
 }
 
 // Exact form: the count pass, one host read of its totals, buffers grown to
@@ -951,7 +968,7 @@ void asohf_hip_destroy(asohf_ctx* ctx) {
                   &ctx->stiles, &ctx->isplit, &ctx->srank, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries, &ctx->items,
                   &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1], &ctx->h_in[2],
                   &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass, &ctx->ord[0], &ctx->ord[1],
-                  &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp};
+                  &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount};
   for (DBuf* b : bufs) b->release();
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);

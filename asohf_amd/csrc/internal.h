@@ -112,6 +112,7 @@ struct Counters {
   unsigned int n_items_eff;       // items the deposit serves (0 after an overflow)
   unsigned int pad[kLine - 12];
   unsigned long long pool_fill[kPoolSubs * kLine / 2];   // sub-pool k at [k * kLine / 2]
+  unsigned long long epool_fill[kPoolSubs * kLine / 2];  // emission lists, same layout
   unsigned int queue[kQueues * kLine];                   // queue q at [q * kLine]
 };
 
@@ -135,6 +136,14 @@ struct BlockTables {
   int* nkeys;                // [blocks]
   unsigned long long* pairs; // [blocks] (particle, grid) members counted by the block
   unsigned* mmax;            // [blocks] max mass of the block, FP32 bits
+  // TSC mode: every emission of the count pass, (tile key << 9) | particle
+  // within the block, so the scatter pass need not enumerate again
+  unsigned* elist;           // kPoolSubs equal sub-pools
+  long long ecap;
+  long long* eoff;           // [blocks] start of the block's list, or -1 (enumerate again)
+  int* ecount;               // [blocks]
+  int use_elist;             // 0 for table mode and for keys of 2^23 or more
+  int pad;
 };
 
 // Arguments of the persistent deposit.

@@ -128,13 +128,21 @@ struct Prefetch {
   }
 };
 
+// emission lists: (key << 9) | particle index within the block
+constexpr int kEmitCap = 4096;
+static_assert(kBinChunk <= 512, "the emission list keeps 9 bits of particle index");
+
 template <int MODE>
 __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, unsigned* cnt, Counters* C,
                                                             BlockTables BT) {
   __shared__ int keys[kBinCap];
   __shared__ unsigned vals[kBinCap];
+  __shared__ unsigned elist_s[MODE == 2 ? kEmitCap : 1];
+  __shared__ int s_ne;
   for (int k = threadIdx.x; k < kBinCap; k += kBinThreads) { keys[k] = -1; vals[k] = 0u; }
+  if (threadIdx.x == 0) s_ne = 0;
   __syncthreads();
+  const bool list = MODE == 2 && BT.use_elist;
   unsigned err = 0;
   unsigned long long pairs = 0;
   float mmax = 0.f;
@@ -156,10 +164,37 @@ __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, uns
           const int h = lds_insert(keys, key);
           if (h >= 0) atomicAdd(&vals[h], 1u);
           else atomicAdd(&cnt[key], 1u);
+          if (list) {
+            const int pos = atomicAdd(&s_ne, 1);
+            if (pos < kEmitCap) elist_s[pos] = ((unsigned)key << 9) | (unsigned)(i - c0);
+          }
         },
         [&](int, int, int, int) { ++pairs; }, err);
   }
   __syncthreads();
+  // the emission list, when it fits, for the scatter pass
+  if (list) {
+    __shared__ long long s_eoff;
+    const int ne = s_ne;
+    if (threadIdx.x == 0) {
+      long long eo = -1;
+      if (ne == 0) {
+        eo = 0;
+      } else if (ne <= kEmitCap) {
+        const int sub = blockIdx.x & (kPoolSubs - 1);
+        const long long subcap = BT.ecap / kPoolSubs;
+        const long long off = (long long)atomicAdd(&C->epool_fill[sub * (kLine / 2)], (unsigned long long)ne);
+        eo = off + ne <= subcap ? sub * subcap + off : -1;
+      }
+      s_eoff = eo;
+      BT.eoff[blockIdx.x] = eo;
+      BT.ecount[blockIdx.x] = ne;
+    }
+    __syncthreads();
+    const long long eo = s_eoff;
+    if (eo >= 0)
+      for (int j = threadIdx.x; j < ne; j += kBinThreads) BT.elist[eo + j] = elist_s[j];
+  }
   // flush the counts, and save the table (compacted) for the scatter pass
   __shared__ int s_n;
   __shared__ long long s_off;
@@ -233,6 +268,7 @@ __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, u
   __shared__ int keys[kBinCap];
   __shared__ unsigned vals[kBinCap];
   __shared__ unsigned long long base[kBinCap];
+  __shared__ int4 parts_s[MODE == 2 ? kBinChunk : 1];
   if (C->overflow) return;   // the entry buffer is too small: the host rebuilds exactly
   for (int k = threadIdx.x; k < kBinCap; k += kBinThreads) { keys[k] = -1; vals[k] = 0u; }
   __syncthreads();
@@ -273,6 +309,28 @@ __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, u
     vals[k] = 0u;
   }
   __syncthreads();
+  // phase C, from the count pass's emission list: the block's particles staged
+  // in LDS, each listed emission written to its reserved slot
+  const long long eo = (MODE == 2 && BT.use_elist) ? BT.eoff[blockIdx.x] : -1;
+  if (MODE == 2 && eo >= 0) {
+    for (int j = threadIdx.x; j < kBinChunk; j += kBinThreads) {
+      const long long i = c0 + j;
+      if (i < A.n)
+        parts_s[j] = make_int4(__float_as_int(A.x[i]), __float_as_int(A.y[i]), __float_as_int(A.z[i]),
+                               __float_as_int(A.m[i]));
+    }
+    __syncthreads();
+    const int ne = BT.ecount[blockIdx.x];
+    for (int j = threadIdx.x; j < ne; j += kBinThreads) {
+      const unsigned v = BT.elist[eo + j];
+      const int key = (int)(v >> 9);
+      const int h = lds_find(keys, key);
+      const unsigned long long slot =
+          h >= 0 ? base[h] + atomicAdd(&vals[h], 1u) : atomicAdd(&cursor[key], 1ull);
+      if (slot < cap) entries[slot] = parts_s[v & 511u];
+    }
+    return;
+  }
   // phase C: the same emissions again, written to their reserved slots
   Prefetch pc(A, c0 + threadIdx.x);
   for (int r = 0; r < kBinPer; ++r) {
