@@ -94,21 +94,37 @@ __host__ __device__ __forceinline__ unsigned bucket_hash(int level, int bx, int 
   return h;
 }
 
+// A hash lookup in two steps, so the first slot load of the next level can be
+// in flight while the current level's candidates are tested.
+struct Probe {
+  uint64_t key;
+  HashSlot h;
+  unsigned s;
+};
+__device__ __forceinline__ Probe probe_start(const Plan& P, int l, float x, float y, float z) {
+  const LevelHash lh = P.lh[l];
+  const int bx = clampi((int)floorf(fminf(fmaxf((x - lh.x0) * lh.inv_bs, -1.f), 1.0e6f)), 0, lh.nb - 1);
+  const int by = clampi((int)floorf(fminf(fmaxf((y - lh.x0) * lh.inv_bs, -1.f), 1.0e6f)), 0, lh.nb - 1);
+  const int bz = clampi((int)floorf(fminf(fmaxf((z - lh.x0) * lh.inv_bs, -1.f), 1.0e6f)), 0, lh.nb - 1);
+  Probe p;
+  p.key = bucket_key(l, bx, by, bz);
+  p.s = bucket_hash(l, bx, by, bz) & P.hmask;
+  p.h = P.hslots[p.s];
+  return p;
+}
+// candidates (start, count) in hrec; the table is at most half full, so an
+// empty slot ends every probe
+__device__ __forceinline__ int2 probe_finish(const Plan& P, Probe p) {
+  for (;;) {
+    if (p.h.key == p.key) return make_int2(p.h.start, p.h.count);
+    if (p.h.key == kHashEmpty) return make_int2(0, 0);
+    p.s = (p.s + 1) & P.hmask;
+    p.h = P.hslots[p.s];
+  }
+}
 // candidate patches of level l whose (inflated) membership box may contain (x,y,z)
 __device__ __forceinline__ int2 bucket_lookup(const Plan& P, int l, float x, float y, float z) {
-  const LevelHash h = P.lh[l];
-  int bx = clampi((int)floorf(fminf(fmaxf((x - h.x0) * h.inv_bs, -1.f), 1.0e6f)), 0, h.nb - 1);
-  int by = clampi((int)floorf(fminf(fmaxf((y - h.x0) * h.inv_bs, -1.f), 1.0e6f)), 0, h.nb - 1);
-  int bz = clampi((int)floorf(fminf(fmaxf((z - h.x0) * h.inv_bs, -1.f), 1.0e6f)), 0, h.nb - 1);
-  const uint64_t key = bucket_key(l, bx, by, bz);
-  unsigned s = bucket_hash(l, bx, by, bz) & P.hmask;
-  // the table is at most half full, so an empty slot ends every probe
-  for (;;) {
-    const HashSlot h = P.hslots[s];
-    if (h.key == key) return make_int2(h.start, h.count);
-    if (h.key == kHashEmpty) return make_int2(0, 0);
-    s = (s + 1) & P.hmask;
-  }
+  return probe_finish(P, probe_start(P, l, x, y, z));
 }
 
 // strict membership test of the reference's box rule on a candidate record
@@ -160,6 +176,9 @@ template <int MODE, class Emit, class Member>
 __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, float x, float y,
                                                    float z, float m, int species, Emit&& emit,
                                                    Member&& member, unsigned& err) {
+  // the first level's hash slot is loaded while the base grid is handled
+  Probe nxt;
+  if (P.nl >= 1) nxt = probe_start(P, 1, x, y, z);
   // Grid fields are copied into registers before the emission loops: the
   // callbacks write memory (atomics, entries), after which the compiler would
   // otherwise reload every field from the descriptor.
@@ -208,7 +227,8 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
   // (checked on the host in FP32, P.nested), a particle in no level-l box is
   // in no deeper box either and the descent stops.
   for (int l = 1; l <= P.nl; ++l) {
-    const int2 se = bucket_lookup(P, l, x, y, z);
+    const int2 se = probe_finish(P, nxt);
+    if (l < P.nl) nxt = probe_start(P, l + 1, x, y, z);
     bool any = false;
     for (int q = 0; q < se.y; ++q) {
       const BoxRec r = P.hrec[se.x + q];
