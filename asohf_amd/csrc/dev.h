@@ -31,13 +31,15 @@ __device__ __forceinline__ void tsc_weights(float xp, float c0, float c1, float 
   b = fabsf(xp - c1) / d; w[1] = 0.75f - b * b;
   b = fabsf(xp - c2) / d; t = 1.5f - b; w[2] = 0.5f * (t * t);
 }
-// the same weights with the division by the cell size replaced by a multiply
-// by its FP32 reciprocal (each b within 1.5 ulp of the reference's)
-__device__ __forceinline__ void tsc_weights_r(float xp, float c0, float c1, float c2, float inv_d, float w[3]) {
-  float b, t;
-  b = fabsf(xp - c0) * inv_d; t = 1.5f - b; w[0] = 0.5f * (t * t);
-  b = fabsf(xp - c1) * inv_d; w[1] = 0.75f - b * b;
-  b = fabsf(xp - c2) * inv_d; t = 1.5f - b; w[2] = 0.5f * (t * t);
+// The same three weights from t = (xp - RAD(I)) / DX alone: for a particle of
+// the cell (|t| <= 1/2 up to rounding) |xp - RAD(I-1)|/DX = 1 + t and
+// |xp - RAD(I+1)|/DX = 1 - t, so the reference's expressions become these
+// polynomials (equal in exact arithmetic, within a few ulp in FP32).
+__device__ __forceinline__ void tsc_weights_t(float t, float w[3]) {
+  const float a = 0.5f - t, b = 0.5f + t;
+  w[0] = 0.5f * (a * a);
+  w[1] = 0.75f - t * t;
+  w[2] = 0.5f * (b * b);
 }
 // derivatives of the three weights with respect to xp
 __device__ __forceinline__ void tsc_dweights(float xp, float c0, float c1, float c2, float d, float dw[3]) {

@@ -687,19 +687,19 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
           const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
           m = __int_as_float(v.w);
           float* w[3] = {wx, wy, wz};
+          // weights from the offset to the centre cell (RAD(I) on the base
+          // grid, MINIRADX(I) on a patch); the indices are the reference's
           if (g == 0) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]);
-              tsc_weights_r(p[a], base_centre(P, a, ci[a] - 1, G.n[a]), base_centre(P, a, ci[a], G.n[a]),
-                            base_centre(P, a, ci[a] + 1, G.n[a]), G.inv_d[a], w[a]);
+              tsc_weights_t((p[a] - base_centre(P, a, ci[a], G.n[a])) * G.inv_d[a], w[a]);
             }
           } else {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               ci[a] = (int)index_quotient(p[a] - G.org[a], G.d[a], G.inv_d[a]);
-              tsc_weights_r(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
-                            minirad(G.pr[a], ci[a] + 1, G.d[a]), G.inv_d[a], w[a]);
+              tsc_weights_t((p[a] - minirad(G.pr[a], ci[a], G.d[a])) * G.inv_d[a], w[a]);
             }
           }
         } else {
