@@ -728,6 +728,7 @@ static asohf_status enqueue_deposit(asohf_ctx* ctx, const PipeArgs& a, hipStream
   HIPCHK(launch_plan_write(a.start, a.ntiles, a.chunk, a.chunk_wide, a.item_off, a.split_off, a.split_rank,
                            const_cast<Item*>(a.D.items), a.split, a.split_tiles, a.D.tile_mass_i, a.C, s),
          "writing items");
+  HIPCHK(launch_zero_split(a.P, a.D, a.red, a.cus, s), "clearing split tiles");
   if (events) HIPCHK(hipEventRecord(ctx->ev[4], s), "event");
   HIPCHK(launch_deposit(a.P, a.D, a.red, a.grid, a.block, s), "deposit kernel");
   if (events) HIPCHK(hipEventRecord(ctx->ev[8], s), "event");

@@ -480,7 +480,7 @@ __global__ void plan_count(const unsigned long long* start, int ntiles, int chun
   tile_chunks(start, t, chunk, chunk_wide, s0, s1, s2, cn, cw);
   const int tot = max(cn + cw, 1);   // an empty tile still writes its zeros
   nitems_tile[t] = (unsigned)tot;
-  nsplit_tile[t] = tot > 1 ? (unsigned)tot : 0u;
+  nsplit_tile[t] = tot > 1 ? 1u : 0u;   // one accumulator tile per split tile
   is_split[t] = tot > 1 ? 1u : 0u;
 }
 
@@ -772,13 +772,17 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
       __syncthreads();
     }
 
-    // ---- split tile: every item stores its partial interior; split_reduce
-    // sums them in item order and writes the cells ----
+    // ---- split tile: every item adds its interior into the tile's
+    // accumulator in global memory (int64 adds are exact, so the sum does not
+    // depend on the order of the items); split_reduce writes the cells ----
     const int nit = D.nitems_tile[t];
     const bool writer = nit == 1;
     if (!writer) {
-      Acc* mine = reinterpret_cast<Acc*>(D.scratch) + ((long long)D.split_slot[t] + (it.flags >> 8)) * T;
-      for (int c = tid; c < T; c += nthr) mine[c] = acc[at(c % TX, (c / TX) % TY, c / (TX * TY))];
+      Acc* dst = reinterpret_cast<Acc*>(D.scratch) + (long long)D.split_slot[t] * T;
+      for (int c = tid; c < T; c += nthr) {
+        const Acc v = acc[at(c % TX, (c / TX) % TY, c / (TX * TY))];
+        if (v != 0) glb_add(dst + c, v);
+      }
     }
     if (writer) {
       float* out = (g == 0 ? D.u1 : D.u11) + G.off;
@@ -808,10 +812,9 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
   }
 }
 
-// Split tiles: the partial interiors of a tile's items summed in item order
-// (fixed order: deterministic in both reductions), normalised and written as
-// the deposit's writer does.  Work unit = (split tile, 256 consecutive cells),
-// so a heavy tile is spread over many workgroups; the exact int64 (FP64 in
+// Split tiles: the accumulated interiors normalised and written as the
+// deposit's writer does.  Work unit = (split tile, 256 consecutive cells), so
+// a heavy tile is spread over many workgroups; the exact int64 (FP64 in
 // atomic mode) tile mass is accumulated per unit.
 template <int RED, int TX, int TY, int TZ>
 __global__ void __launch_bounds__(256) split_reduce(Plan P, DepArgs D) {
@@ -833,15 +836,12 @@ __global__ void __launch_bounds__(256) split_reduce(Plan P, DepArgs D) {
     const int tx = local % G.nt[0], ty = (local / G.nt[0]) % G.nt[1], tz = local / (G.nt[0] * G.nt[1]);
     const int i0 = tx * TX + 1, j0 = ty * TY + 1, k0c = tz * TZ + 1;
     const int ex = min(TX, G.n[0] - i0 + 1), ey = min(TY, G.n[1] - j0 + 1), ez = min(TZ, G.n[2] - k0c + 1);
-    const int nit = D.nitems_tile[t];
     const Acc* part = reinterpret_cast<const Acc*>(D.scratch) + (long long)D.split_slot[t] * T;
     long long msi = 0;
     double msd = 0.0;
     const int a = c % TX, b = (c / TX) % TY, cz = c / (TX * TY);
     if (c < T && a < ex && b < ey && cz < ez) {
-      Acc v = 0;
-#pragma unroll 4
-      for (int k = 0; k < nit; ++k) v += part[(long long)k * T + c];
+      const Acc v = part[c];
       if (RED == 0) msi = (long long)v;
       else msd = (double)v;
       float f = RED == 0 ? (float)((double)v * inv_scale) : (float)v;
@@ -859,6 +859,17 @@ __global__ void __launch_bounds__(256) split_reduce(Plan P, DepArgs D) {
       if (threadIdx.x == 0 && msd != 0.0) atomicAdd(D.tile_mass_d + t, msd);
     }
   }
+}
+
+// clears the accumulator tiles of this call's split tiles (their count is on
+// the device); a kernel, not a memset, so the build can be captured
+__global__ void __launch_bounds__(256) zero_split(DepArgs D, int T, int elem_words) {
+  if (D.C->overflow) return;
+  const unsigned long long n = *D.n_split_tiles * (unsigned long long)T * (unsigned long long)elem_words;
+  unsigned* p = reinterpret_cast<unsigned*>(D.scratch);
+  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    p[i] = 0u;
 }
 
 // per-grid mass in a fixed order; FP64 (int64-exact in deterministic mode)
@@ -1330,6 +1341,12 @@ static hipError_t split_shape(const Plan& P, const DepArgs& D, int grid, hipStre
   else if (t[0] == 16 && t[1] == 16 && t[2] == 8) split_reduce<RED, 16, 16, 8><<<grid, 256, 0, s>>>(P, D);
   else if (t[0] == 16 && t[1] == 16 && t[2] == 16) split_reduce<RED, 16, 16, 16><<<grid, 256, 0, s>>>(P, D);
   else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_split(const Plan& P, const DepArgs& D, int reduction, int cus, hipStream_t s) {
+  const int T = P.tile[0] * P.tile[1] * P.tile[2];
+  zero_split<<<(cus > 1 ? cus : 1) * 8, 256, 0, s>>>(D, T, reduction == 0 ? 2 : 1);
   return hipGetLastError();
 }
 

@@ -30,6 +30,7 @@ hipError_t launch_gate(const unsigned long long* tot_entries, const unsigned lon
                        unsigned long long cap_items, unsigned long long cap_split, Counters* C, hipStream_t s);
 bool tile_shape_supported(const int t[3]);
 hipError_t launch_deposit(const Plan& P, const DepArgs& D, int reduction, int grid, int block, hipStream_t s);
+hipError_t launch_zero_split(const Plan& P, const DepArgs& D, int reduction, int cus, hipStream_t s);
 hipError_t launch_split_reduce(const Plan& P, const DepArgs& D, int reduction, int cus, hipStream_t s);
 hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, double* grid_mass, hipStream_t s);
 hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
