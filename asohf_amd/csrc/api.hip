@@ -1208,6 +1208,23 @@ asohf_status asohf_hip_autotune(asohf_ctx* ctx, const asohf_geom* geom, int64_t 
   return ASOHF_OK;
 }
 
+asohf_status asohf_hip_release_workspaces(asohf_ctx* ctx) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  HIPCHK(hipStreamSynchronize(ctx->stream), "waiting for the context's stream");
+  if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+  ctx->gexec = nullptr;
+  DBuf* bufs[] = {&ctx->pool, &ctx->pool_off, &ctx->pool_nk, &ctx->blk_pairs, &ctx->blk_mmax, &ctx->nsplit,
+                  &ctx->item_off, &ctx->split_off, &ctx->cnt, &ctx->start, &ctx->cursor, &ctx->bsum,
+                  &ctx->stiles, &ctx->isplit, &ctx->srank, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries,
+                  &ctx->items, &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1],
+                  &ctx->h_in[2], &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass, &ctx->ord[0],
+                  &ctx->ord[1], &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount};
+  for (DBuf* b : bufs) b->release();
+  ctx->sized = false;   // the next batched call sizes its buffers again
+  return ASOHF_OK;
+}
+
 asohf_status asohf_hip_get_stats(const asohf_ctx* cctx, asohf_stats* out) {
   if (!cctx || !out) return ASOHF_ERR_INVALID;
   asohf_ctx* ctx = const_cast<asohf_ctx*>(cctx);

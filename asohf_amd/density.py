@@ -87,6 +87,10 @@ class DensityBuilder:
         _lib.check(None, L.asohf_hip_create(C.byref(h), self.device))
         self._h = h
 
+    def release_workspaces(self):
+        """Free the device workspaces kept between calls (the plan stays)."""
+        _lib.check(self._h, _lib.load().asohf_hip_release_workspaces(self._h))
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.load().asohf_hip_destroy(self._h)

@@ -219,6 +219,9 @@ def main() -> int:
         bld.density(g, x, y, z, m, snap.npart_esp, **kw)
         torch.cuda.synchronize()
 
+    # the forward workspaces (sized by the largest mode above) are not needed by
+    # the adjoint; free them so the adjoint's own buffers fit at any size
+    bld.release_workspaces()
     if args.mode == "tsc":
         # the hand-written adjoint: d(sum(w*rho))/d(positions, masses)
         gen = torch.Generator(device=dev).manual_seed(spec.seed)

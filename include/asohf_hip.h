@@ -169,6 +169,14 @@ asohf_status asohf_hip_autotune(asohf_ctx *ctx, const asohf_geom *geom, int64_t 
 
 asohf_status asohf_hip_get_stats(const asohf_ctx *ctx, asohf_stats *out);
 
+/* Frees the device workspaces the context keeps between calls (binned
+ * entries, work items, block tables, emission lists, split tiles, the
+ * adjoint's sort buffers, the captured graph); the geometry plan stays, and
+ * the next call allocates what it needs again.  No counterpart in the
+ * reference, which holds no device memory: for callers that alternate the
+ * density build with other large device work. */
+asohf_status asohf_hip_release_workspaces(asohf_ctx *ctx);
+
 /* Version string of the library. */
 const char *asohf_hip_version(void);
 

@@ -42,6 +42,7 @@ def test_null_context_is_rejected():
     L = _lib.load()
     assert L.asohf_hip_density(None, None, 0, None, None, None, None, None, None, None, None, None) == -1
     assert L.asohf_hip_get_stats(None, None) == -1
+    assert L.asohf_hip_release_workspaces(None) == -1
 
 
 def test_synthetic_hierarchy_obeys_mesh_rules():

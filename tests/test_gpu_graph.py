@@ -135,3 +135,24 @@ def test_buffers_regrow_on_larger_input(use_graph):
         assert abs(gm[0] - total) <= 1e-12 * total
         assert b.stats()["n_particles"] == k
     b.close()
+
+
+def test_release_workspaces_then_rebuild():
+    """Freeing the context's workspaces (and its graph) between calls: the
+    next calls size them again and give the same bits."""
+    snap = small(seed=9)
+    b = fresh()
+    x, y, z, m = to_dev(snap)
+    r0 = b.density(snap.geom, x, y, z, m, mode="tsc")
+    torch.cuda.synchronize()
+    ref = [t.cpu().numpy().copy() for t in (r0.u1, r0.u11, r0.grid_mass)]
+    for use_graph in (True, False, True):
+        print(f"case release graph={use_graph}", flush=True)
+        b.release_workspaces()
+        for _ in range(2):
+            r = b.density(snap.geom, x, y, z, m, mode="tsc", use_graph=use_graph,
+                          u1=r0.u1, u11=r0.u11, grid_mass=r0.grid_mass)
+            torch.cuda.synchronize()
+            for a, want in zip((r.u1, r.u11, r.grid_mass), ref):
+                assert np.array_equal(a.cpu().numpy(), want)
+    b.close()
