@@ -175,6 +175,10 @@ def main() -> int:
                      "n_pairs": int(n_pairs), "n_entries": int(n_entries)},
     }
     u1_gpu = r.u1.cpu().numpy()
+    # size-independent check of the timed result: every particle's mass is
+    # deposited exactly once on the base grid (FP64 grid sum vs FP64 total)
+    total = float(m.double().sum())
+    out["checks"] = {"base_grid_mass_rel_err": abs(float(r.grid_mass[0]) - total) / total if total else 0.0}
 
     if args.modes_steps > 0:
         # every mode x reduction on the same snapshot (the headline line above is one of them)
