@@ -640,8 +640,8 @@ static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Part
   HIPCHK(ctx->pool_nk.ensure(sizeof(int) * std::max<long long>(nblk, 1)), "allocating block tables");
   HIPCHK(ctx->blk_pairs.ensure(sizeof(unsigned long long) * std::max<long long>(nblk, 1)), "allocating block sums");
   HIPCHK(ctx->blk_mmax.ensure(sizeof(unsigned) * std::max<long long>(nblk, 1)), "allocating block sums");
-  // emission lists (TSC): about four per particle until a call has measured them
-  a.BT.use_elist = (o.mode == ASOHF_MODE_TSC && 2ll * ntiles < (1ll << 23)) ? 1 : 0;
+  // emission lists: about four per particle until a call has measured them
+  a.BT.use_elist = 2ll * ntiles < (1ll << 23) ? 1 : 0;
   if (a.BT.use_elist) {
     const long long ecap = std::max<long long>(ctx->epool_want + ctx->epool_want / 8, 4 * A.n + 4096);
     HIPCHK(ctx->elist.ensure(sizeof(unsigned) * ecap), "allocating emission lists");

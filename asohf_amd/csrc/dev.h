@@ -150,6 +150,24 @@ __device__ __forceinline__ int4 pack_table(int ix, int jy, int kz, int irk, floa
   e.w = 0;
   return e;
 }
+// The table-mode entries of a particle that enumerate_particle produces,
+// rebuilt from the particle and the grid alone (same expressions): the base
+// cell INT((XP-XL)/DX)+1 (range checked by the enumeration), and on a patch
+// FLOOR((XP-XLFIX)/DXPA)+1 with the stencil level max(IR - species, 0).
+__device__ __forceinline__ int4 table_base_entry(const Plan& P, float x, float y, float z, float m) {
+  const GridDesc& B = P.grids[0];
+  const int ix = (int)index_quotient(x - P.xl, B.d[0], B.inv_d[0]) + 1;
+  const int jy = (int)index_quotient(y - P.xl, B.d[1], B.inv_d[1]) + 1;
+  const int kz = (int)index_quotient(z - P.xl, B.d[2], B.inv_d[2]) + 1;
+  return pack_table(ix, jy, kz, 0, m);
+}
+__device__ __forceinline__ int4 table_patch_entry(const GridDesc& G, float x, float y, float z, float m,
+                                                  int species) {
+  const int ix = (int)floorf(index_quotient(x - G.org[0], G.d[0], G.inv_d[0])) + 1;
+  const int jy = (int)floorf(index_quotient(y - G.org[1], G.d[1], G.inv_d[1])) + 1;
+  const int kz = (int)floorf(index_quotient(z - G.org[2], G.d[2], G.inv_d[2])) + 1;
+  return pack_table(ix, jy, kz, max(G.level - species, 0), m);
+}
 __device__ __forceinline__ void unpack_table(int4 e, int& ix, int& jy, int& kz, int& irk, float& m) {
   ix = (int)(short)(e.x & 0xffff);
   jy = e.x >> 16;

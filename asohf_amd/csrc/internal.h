@@ -136,8 +136,8 @@ struct BlockTables {
   int* nkeys;                // [blocks]
   unsigned long long* pairs; // [blocks] (particle, grid) members counted by the block
   unsigned* mmax;            // [blocks] max mass of the block, FP32 bits
-  // TSC mode: every emission of the count pass, (tile key << 9) | particle
-  // within the block, so the scatter pass need not enumerate again
+  // every emission of the count pass, (tile key << 9) | particle within the
+  // block, so the scatter pass need not enumerate again
   unsigned* elist;           // kPoolSubs equal sub-pools
   long long ecap;
   long long* eoff;           // [blocks] start of the block's list, or -1 (enumerate again)

@@ -137,12 +137,12 @@ __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, uns
                                                             BlockTables BT) {
   __shared__ int keys[kBinCap];
   __shared__ unsigned vals[kBinCap];
-  __shared__ unsigned elist_s[MODE == 2 ? kEmitCap : 1];
+  __shared__ unsigned elist_s[kEmitCap];
   __shared__ int s_ne;
   for (int k = threadIdx.x; k < kBinCap; k += kBinThreads) { keys[k] = -1; vals[k] = 0u; }
   if (threadIdx.x == 0) s_ne = 0;
   __syncthreads();
-  const bool list = MODE == 2 && BT.use_elist;
+  const bool list = BT.use_elist;
   unsigned err = 0;
   unsigned long long pairs = 0;
   float mmax = 0.f;
@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, u
   __shared__ int keys[kBinCap];
   __shared__ unsigned vals[kBinCap];
   __shared__ unsigned long long base[kBinCap];
-  __shared__ int4 parts_s[MODE == 2 ? kBinChunk : 1];
+  __shared__ int4 parts_s[kBinChunk];
   if (C->overflow) return;   // the entry buffer is too small: the host rebuilds exactly
   for (int k = threadIdx.x; k < kBinCap; k += kBinThreads) { keys[k] = -1; vals[k] = 0u; }
   __syncthreads();
@@ -311,8 +311,8 @@ __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, u
   __syncthreads();
   // phase C, from the count pass's emission list: the block's particles staged
   // in LDS, each listed emission written to its reserved slot
-  const long long eo = (MODE == 2 && BT.use_elist) ? BT.eoff[blockIdx.x] : -1;
-  if (MODE == 2 && eo >= 0) {
+  const long long eo = BT.use_elist ? BT.eoff[blockIdx.x] : -1;
+  if (eo >= 0) {
     for (int j = threadIdx.x; j < kBinChunk; j += kBinThreads) {
       const long long i = c0 + j;
       if (i < A.n)
@@ -327,7 +327,17 @@ __global__ void __launch_bounds__(kBinThreads) scatter_kernel(Plan P, Parts A, u
       const int h = lds_find(keys, key);
       const unsigned long long slot =
           h >= 0 ? base[h] + atomicAdd(&vals[h], 1u) : atomicAdd(&cursor[key], 1ull);
-      if (slot < cap) entries[slot] = parts_s[v & 511u];
+      const int li = (int)(v & 511u);
+      int4 e = parts_s[li];
+      if (MODE == 1) {
+        // the table-mode entry: cell indices in the tile's grid
+        const float x = __int_as_float(e.x), y = __int_as_float(e.y), z = __int_as_float(e.z),
+                    m = __int_as_float(e.w);
+        const int g = P.tile_grid[key >> 1];
+        e = g == 0 ? table_base_entry(P, x, y, z, m)
+                   : table_patch_entry(P.grids[g], x, y, z, m, species_of(A, c0 + li));
+      }
+      if (slot < cap) entries[slot] = e;
     }
     return;
   }
