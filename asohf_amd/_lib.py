@@ -23,7 +23,7 @@ PATH_BATCHED, PATH_PLAIN = 0, 1
 EXPORTS = ("asohf_hip_create", "asohf_hip_destroy", "asohf_hip_last_error", "asohf_hip_density",
            "asohf_hip_density_host", "asohf_hip_density_adjoint", "asohf_hip_membership",
            "asohf_hip_get_tune", "asohf_hip_set_tune", "asohf_hip_autotune", "asohf_hip_get_stats",
-           "asohf_hip_release_workspaces", "asohf_hip_version")
+           "asohf_hip_release_workspaces", "asohf_hip_species", "asohf_hip_version")
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
@@ -101,6 +101,8 @@ def load():
                                          C.POINTER(Tune)]
         L.asohf_hip_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.asohf_hip_release_workspaces.argtypes = [vp]
+        L.asohf_hip_species.argtypes = [vp, _ip, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int64,
+                                        vp, vp, vp, vp, vp, vp, _ip, vp]
         for nm in EXPORTS:
             if nm not in ("asohf_hip_destroy", "asohf_hip_last_error", "asohf_hip_version"):
                 getattr(L, nm).restype = C.c_int32

@@ -173,6 +173,7 @@ struct asohf_ctx {
   DBuf cnt, start, cursor, bsum, counters, stiles, isplit, srank, nitems, split, tmass, entries, items, scratch;
   DBuf flags, idx, pool, pool_off, pool_nk, blk_pairs, blk_mmax, nsplit, item_off, split_off;
   DBuf elist, eoff, ecount;
+  DBuf sp_k[4], sp_dens, sp_cnt, sp_tmp;   // species partition
   long long pool_want = 0;   // pairs the last count pass needed
   long long epool_want = 0;  // emission-list words the last count pass needed
   bool sized = false;        // the entry/item/scratch buffers were sized by an exact call
@@ -969,7 +970,8 @@ void asohf_hip_destroy(asohf_ctx* ctx) {
                   &ctx->stiles, &ctx->isplit, &ctx->srank, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries, &ctx->items,
                   &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1], &ctx->h_in[2],
                   &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass, &ctx->ord[0], &ctx->ord[1],
-                  &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount};
+                  &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount, &ctx->sp_k[0],
+                  &ctx->sp_k[1], &ctx->sp_k[2], &ctx->sp_k[3], &ctx->sp_dens, &ctx->sp_cnt, &ctx->sp_tmp};
   for (DBuf* b : bufs) b->release();
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1208,6 +1210,42 @@ asohf_status asohf_hip_autotune(asohf_ctx* ctx, const asohf_geom* geom, int64_t 
   return ASOHF_OK;
 }
 
+asohf_status asohf_hip_species(asohf_ctx* ctx, const int32_t nmax[3], const float d[3], float rodo, float re0,
+                               int64_t n, const float* rx, const float* ry, const float* rz, const float* m,
+                               int32_t* species, int64_t* order, int32_t* npart_esp, void* stream) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  if (!nmax || !d || !npart_esp || n < 0 || (n > 0 && (!rx || !ry || !rz || !m || !order)))
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL argument");
+  for (int a = 0; a < 3; ++a) {
+    if (nmax[a] < 1 || nmax[a] > 4096) return fail(ctx, ASOHF_ERR_INVALID, "NMAX outside 1..4096");
+    if (!(d[a] > 0.f) || !std::isfinite(d[a])) return fail(ctx, ASOHF_ERR_INVALID, "cell size must be > 0");
+  }
+  if (n >= (1ll << 32)) return fail(ctx, ASOHF_ERR_CAPACITY, "2^32 or more particles");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t s = (hipStream_t)stream;
+  const long long nc = (long long)nmax[0] * nmax[1] * nmax[2];
+  // BAS = DX*DY*DZ*RODO*RE0**3 in FP32, left to right (reader.f:203)
+  const float bas = (((d[0] * d[1]) * d[2]) * rodo) * powi_f(re0, 3);
+  const size_t b4 = sizeof(unsigned) * (size_t)std::max<int64_t>(n, 1);
+  for (int k = 0; k < 4; ++k) HIPCHK(ctx->sp_k[k].ensure(b4), "allocating the species sort");
+  HIPCHK(ctx->sp_dens.ensure(sizeof(float) * (size_t)nc), "allocating the species grid");
+  HIPCHK(ctx->sp_cnt.ensure(sizeof(unsigned) * kNEsp), "allocating the species counts");
+  size_t tb = 0;
+  HIPCHK(species_sort_bytes((unsigned long long)std::max<int64_t>(n, 1), tb), "sizing the species sort");
+  HIPCHK(ctx->sp_tmp.ensure(std::max<size_t>(tb, 1)), "allocating the species sort");
+  HIPCHK(launch_species(nmax, d, bas, n, rx, ry, rz, m, species, (long long*)order, ctx->sp_cnt.as<unsigned>(),
+                        ctx->sp_dens.as<float>(), ctx->sp_k[0].as<unsigned>(), ctx->sp_k[1].as<unsigned>(),
+                        ctx->sp_k[2].as<unsigned>(), ctx->sp_k[3].as<unsigned>(), ctx->sp_tmp.p,
+                        ctx->sp_tmp.cap, ctx->cus, s),
+         "species kernels");
+  unsigned cnt[kNEsp];
+  HIPCHK(hipMemcpyAsync(cnt, ctx->sp_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, s), "reading the counts");
+  HIPCHK(hipStreamSynchronize(s), "species partition");
+  for (int k = 0; k < kNEsp; ++k) npart_esp[k] = (int32_t)cnt[k];
+  return ASOHF_OK;
+}
+
 asohf_status asohf_hip_release_workspaces(asohf_ctx* ctx) {
   if (!ctx) return ASOHF_ERR_INVALID;
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
@@ -1219,7 +1257,9 @@ asohf_status asohf_hip_release_workspaces(asohf_ctx* ctx) {
                   &ctx->stiles, &ctx->isplit, &ctx->srank, &ctx->nitems, &ctx->split, &ctx->tmass, &ctx->entries,
                   &ctx->items, &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1],
                   &ctx->h_in[2], &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass, &ctx->ord[0],
-                  &ctx->ord[1], &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount};
+                  &ctx->ord[1], &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount,
+                  &ctx->sp_k[0], &ctx->sp_k[1], &ctx->sp_k[2], &ctx->sp_k[3], &ctx->sp_dens, &ctx->sp_cnt,
+                  &ctx->sp_tmp};
   for (DBuf* b : bufs) b->release();
   ctx->sized = false;   // the next batched call sizes its buffers again
   return ASOHF_OK;

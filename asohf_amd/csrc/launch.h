@@ -36,6 +36,11 @@ hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, doub
 hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
                           float* dx, float* dy, float* dz, const unsigned* order, int cus, hipStream_t s);
 hipError_t launch_cell_keys(const Plan& P, const Parts& A, unsigned* key, unsigned* idx, int cus, hipStream_t s);
+hipError_t species_sort_bytes(unsigned long long n, size_t& bytes);
+hipError_t launch_species(const int nmax[3], const float d[3], float bas, long long n, const float* x,
+                          const float* y, const float* z, const float* m, int* species, long long* order,
+                          unsigned* counts, float* dens, unsigned* k0, unsigned* k1, unsigned* k2, unsigned* k3,
+                          void* temp, size_t temp_bytes, int cus, hipStream_t s);
 hipError_t sort_cell_pairs(void* temp, size_t& temp_bytes, const unsigned* key_in, unsigned* key_out,
                            const unsigned* idx_in, unsigned* idx_out, unsigned long long n, int end_bit,
                            hipStream_t s);

@@ -87,6 +87,27 @@ class DensityBuilder:
         _lib.check(None, L.asohf_hip_create(C.byref(h), self.device))
         self._h = h
 
+    def species(self, x, y, z, m, nmax=(128, 128, 128), d=(1.0, 1.0, 1.0), rodo=1.0, re0=1.0,
+                want_species=True):
+        """SORT_DM_PARTICLES_LOCALDENSITY (reader.f:140-336): each particle's
+        kernel class from the local density on the reference's NMAX^3 grid of
+        cell d, and the stable partition by class.  Returns (species int32
+        tensor or None, order int64 tensor: original index at each position,
+        npart_esp numpy int32[9]).  Apply with x[order] etc."""
+        n = _check_particles(x, y, z, m)
+        dev = x.device
+        sp = torch.empty(max(n, 1), dtype=torch.int32, device=dev) if want_species else None
+        order = torch.empty(max(n, 1), dtype=torch.int64, device=dev)
+        esp = np.zeros(N_ESP, np.int32)
+        nm = np.ascontiguousarray(nmax, np.int32)
+        dd = np.ascontiguousarray(d, np.float32)
+        st = _lib.load().asohf_hip_species(
+            self._h, _np_ptr(nm, C.c_int32), dd.ctypes.data_as(C.POINTER(C.c_float)), C.c_float(rodo),
+            C.c_float(re0), n, _ptr(x), _ptr(y), _ptr(z), _ptr(m), _ptr(sp) if sp is not None else None,
+            _ptr(order), _np_ptr(esp, C.c_int32), C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+        _lib.check(self._h, st)
+        return (sp[:n] if sp is not None else None), order[:n], esp
+
     def release_workspaces(self):
         """Free the device workspaces kept between calls (the plan stays)."""
         _lib.check(self._h, _lib.load().asohf_hip_release_workspaces(self._h))

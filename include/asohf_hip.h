@@ -169,6 +169,22 @@ asohf_status asohf_hip_autotune(asohf_ctx *ctx, const asohf_geom *geom, int64_t 
 
 asohf_status asohf_hip_get_stats(const asohf_ctx *ctx, asohf_stats *out);
 
+/* Kernel classes of the particles by local density and their stable
+ * partition: SORT_DM_PARTICLES_LOCALDENSITY (reader.f:140-336), the reader's
+ * default species rule that feeds NPART_ESP of DENSITY (asohf.f:422-444).
+ * nmax[3] = the reference's NMAX, NMAY, NMAZ (its grid for this rule, of
+ * cells d[3] = DX, DY, DZ, centred on 0); rodo, re0 = RODO, RE0 (asohf.f:
+ * 351-358).  On device arrays: species[n] (may be NULL) receives each
+ * particle's class, order[n] the original index (0-based) of the particle at
+ * each position of the partition (apply it to every particle array, as the
+ * reference reorders RXPA..ORIPA); npart_esp[9] (host) the class counts.
+ * Masses are summed per cell in FP32 in particle order, as the reference's
+ * loop without OpenMP does.  Synchronises before returning. */
+asohf_status asohf_hip_species(asohf_ctx *ctx, const int32_t nmax[3], const float d[3], float rodo, float re0,
+                               int64_t n, const float *rx, const float *ry, const float *rz,
+                               const float *m, int32_t *species, int64_t *order, int32_t *npart_esp,
+                               void *stream);
+
 /* Frees the device workspaces the context keeps between calls (binned
  * entries, work items, block tables, emission lists, split tiles, the
  * adjoint's sort buffers, the captured graph); the geometry plan stays, and

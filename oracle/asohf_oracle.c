@@ -1031,3 +1031,62 @@ double orc_loss_one_b64(const orc_geom *g, float xp, float yp, float zp, float m
   free(rs.own);
   return L;
 }
+
+/* ------------------------------------------------------------------ */
+/* Species by local density: SORT_DM_PARTICLES_LOCALDENSITY            */
+/* (reader.f:140-336), FP32, sequential (the reference without OpenMP). */
+/*   grid: the reference's NMAX x NMAY x NMAZ cells of DX, centred on 0  */
+/*   (XMIN = -NMAX*DX/2, reader.f:183-185); NGP cell INT((X-XMIN)/DX)+1  */
+/*   clamped to [1, NMAX] (reader.f:190-200); mass sums in particle      */
+/*   order; divided by BAS = DX*DY*DZ*RODO*RE0**3 (reader.f:203-212);     */
+/*   species = MAX(MIN(INT(LOG(D)/LOG(8.0)), N_ESP-1), 0) for D > 0,     */
+/*   else 0 (reader.f:219-236); then the stable partition by species     */
+/*   (reader.f:244-262).  order[k] = original index (0-based) of the     */
+/*   particle at position k.                                             */
+/* ------------------------------------------------------------------ */
+static int species_cell(float v, float vmin, float d, int n) {
+  int i = (int)((v - vmin) / d) + 1;
+  if (i < 1) i = 1;
+  if (i > n) i = n;
+  return i - 1;
+}
+
+int orc_species32(int nx, int ny, int nz, float dx, float dy, float dz, float rodo, float re0, int64_t n,
+                  const float *x, const float *y, const float *z, const float *m, int32_t *species,
+                  int32_t *npart_esp, int64_t *order) {
+  const size_t nc = (size_t)nx * ny * nz;
+  float *dens = (float *)calloc(nc > 0 ? nc : 1, sizeof(float));
+  if (!dens) return -1;
+  const float xmin = -(((float)nx * dx) / 2.0f);
+  const float ymin = -(((float)ny * dy) / 2.0f);
+  const float zmin = -(((float)nz * dz) / 2.0f);
+  for (int64_t i = 0; i < n; ++i) {
+    const int ix = species_cell(x[i], xmin, dx, nx), jy = species_cell(y[i], ymin, dy, ny),
+              kz = species_cell(z[i], zmin, dz, nz);
+    dens[(size_t)ix + (size_t)nx * ((size_t)jy + (size_t)ny * kz)] += m[i];
+  }
+  const float bas = (((dx * dy) * dz) * rodo) * powi_f(re0, 3);
+  for (size_t c = 0; c < nc; ++c) dens[c] = dens[c] / bas;
+  const float l8 = logf(8.0f);
+  for (int64_t i = 0; i < n; ++i) {
+    const int ix = species_cell(x[i], xmin, dx, nx), jy = species_cell(y[i], ymin, dy, ny),
+              kz = species_cell(z[i], zmin, dz, nz);
+    const float b = dens[(size_t)ix + (size_t)nx * ((size_t)jy + (size_t)ny * kz)];
+    int s = 0;
+    if (b > 0.0f) {
+      s = (int)(logf(b) / l8);
+      if (s > ORC_N_ESP - 1) s = ORC_N_ESP - 1;
+      if (s < 0) s = 0;
+    }
+    species[i] = s;
+  }
+  free(dens);
+  int64_t k = 0;
+  for (int s = 0; s < ORC_N_ESP; ++s) {
+    int64_t c0 = k;
+    for (int64_t i = 0; i < n; ++i)
+      if (species[i] == s) order[k++] = i;
+    npart_esp[s] = (int32_t)(k - c0);
+  }
+  return 0;
+}

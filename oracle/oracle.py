@@ -227,6 +227,51 @@ def stencil_work_a(snap) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # the reference's own routines (oracle/_ref)
 # ---------------------------------------------------------------------------
+def species32(x, y, z, m, nmax: int, dx: float, rodo: float, re0: float):
+    """SORT_DM_PARTICLES_LOCALDENSITY (reader.f:140-336): (species[n],
+    npart_esp[9], order[n]) on an nmax^3 grid of cell dx (the reference's
+    NMAX = NMAY = NMAZ; DX = DY = DZ)."""
+    x, y, z, m = (np.ascontiguousarray(a, np.float32) for a in (x, y, z, m))
+    n = len(x)
+    sp = np.zeros(max(n, 1), np.int32)
+    esp = np.zeros(9, np.int32)
+    order = np.zeros(max(n, 1), np.int64)
+    L = lib()
+    L.orc_species32.restype = C.c_int
+    rc = L.orc_species32(C.c_int(nmax), C.c_int(nmax), C.c_int(nmax), C.c_float(dx), C.c_float(dx),
+                         C.c_float(dx), C.c_float(rodo), C.c_float(re0), C.c_int64(n), _p(x, _fp),
+                         _p(y, _fp), _p(z, _fp), _p(m, _fp), _p(sp, _ip), _p(esp, _ip),
+                         _p(order, C.POINTER(C.c_int64)))
+    if rc != 0:
+        raise MemoryError("oracle species grid")
+    return sp[:n], esp, order[:n]
+
+
+def run_ref_species(x, y, z, m, nx: int, lado0: float, rodo: float, re0: float):
+    """The reference's SORT_DM_PARTICLES_LOCALDENSITY (its compile-time
+    NMAX = 128 grid; DX from MALLA for this NX and LADO0): (dx, npart_esp,
+    order[n] 0-based)."""
+    x, y, z, m = (np.ascontiguousarray(a, np.float32) for a in (x, y, z, m))
+    n = len(x)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([3, nx, nx, nx, 0, n, 2, 0], np.int32).tofile(f)
+            np.zeros(1, np.int32).tofile(f)          # NPATCH(0)
+            np.zeros(9, np.int32).tofile(f)          # NPART_ESP (output)
+            np.array([lado0, rodo, re0, 1.0], np.float32).tofile(f)
+            for a in (x, y, z, m):
+                a.tofile(f)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([REF_BIN, fin, fout], check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, env=env)
+        raw = np.fromfile(fout, np.uint8)
+    d = np.frombuffer(raw[:12].tobytes(), np.float32)
+    esp = np.frombuffer(raw[12:48].tobytes(), np.int32)
+    order = np.frombuffer(raw[48:48 + 4 * n].tobytes(), np.int32).astype(np.int64) - 1
+    return float(d[0]), esp.copy(), order
+
+
 def ref_available() -> bool:
     return os.path.exists(REF_BIN)
 

@@ -6,6 +6,10 @@
 *     without OpenMP, on one input file and writes the fields they fill.
 *       mode 1: SUBROUTINE DENSITY            (grids.f:710-994)
 *       mode 2: SUBROUTINE INTERPOLATE_DENSITY_TSC (grids.f:1070-1398)
+*       mode 3: SUBROUTINE SORT_DM_PARTICLES_LOCALDENSITY (reader.f:140-336),
+*               with RODO = SCAL(2) and RE0 = SCAL(3); writes DX, DY, DZ,
+*               NPART_ESP(0:8) and ORIPA(1:N) (the original index of the
+*               particle at each position after the partition)
 *     The grid constants come from the reference's MALLA (grids.f:2509)
 *     called with LADO = LADO0 - LADO0/NX as asohf.f:323 does.
 *
@@ -107,6 +111,24 @@
       T=0.0
       ZETA=0.0
 
+      IF (MODE.EQ.3) THEN
+       ALLOCATE(U2DM(PARTI),U3DM(PARTI),U4DM(PARTI),ORIPA(PARTI))
+       DO I=1,N
+        U2DM(I)=0.0
+        U3DM(I)=0.0
+        U4DM(I)=0.0
+        ORIPA(I)=I
+       END DO
+       CALL SORT_DM_PARTICLES_LOCALDENSITY(N,NPART_ESP,0,0,
+     &                                     SCAL(2),SCAL(3))
+       OPEN(11,FILE=FOUT,ACCESS='STREAM',FORM='UNFORMATTED',
+     &      STATUS='REPLACE',ACTION='WRITE')
+       WRITE(11) DX,DY,DZ
+       WRITE(11) NPART_ESP
+       WRITE(11) (ORIPA(I),I=1,N)
+       CLOSE(11)
+       STOP
+      END IF
       IF (MODE.EQ.1) THEN
        CALL DENSITY(0,NX,NY,NZ,NL,NPATCH,PARE,PATCHNX,PATCHNY,
      &              PATCHNZ,PATCHX,PATCHY,PATCHZ,PATCHRX,PATCHRY,
