@@ -19,6 +19,7 @@ Prints ONE JSON line on stdout; progress goes to stderr.
   modes      ms/step of every mode (TSC, kernel table) x reduction
              (deterministic, atomic) on the same snapshot
   per_level_ms  builds of the hierarchy cut at level L = 0..NL, differenced
+  species    the species partition that feeds NPART_ESP (reader.f:140-336)
   adjoint    the hand-written backward (gradients of sum(w * rho))
   plain_gpu  the same build as one launch per grid with global FP32 atomics
   cpu_baseline  the oracle's OpenMP restatement of the reference algorithm
@@ -64,6 +65,7 @@ def main() -> int:
                     help="seconds of CPU work per sample when the whole CPU build is too long")
     ap.add_argument("--modes-steps", type=int, default=5, help="steps per mode x reduction (0: skip)")
     ap.add_argument("--no-levels", dest="levels", action="store_false", help="skip the per-level timing")
+    ap.add_argument("--species-steps", type=int, default=5, help="calls of the species partition (0: skip)")
     ap.add_argument("--plain-max-tests", type=float, default=5e12,
                     help="skip the plain form above this many particle-box tests")
     ap.add_argument("--tune", default="", help="tx,ty,tz,chunk,waves_per_cu[,block]: fixed tuning, no autotune")
@@ -222,6 +224,32 @@ def main() -> int:
         log(f"[bench] per level (cumulative ms): {cum}")
         bld.density(g, x, y, z, m, snap.npart_esp, **kw)
         torch.cuda.synchronize()
+
+    if args.species_steps > 0:
+        # the species partition that feeds NPART_ESP (reader.f:140-336), on a
+        # grid of NMAX = NX cells per axis (a build sized for this run)
+        from asohf_amd.geometry import background
+        rodo, re0 = (float(v) for v in background(spec.h, spec.omega_m, 0.0))
+        skw = dict(nmax=(g.nx,) * 3, d=(float(g.dx),) * 3, rodo=rodo, re0=re0)
+        bld.species(x, y, z, m, **skw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.species_steps):
+            _, order, esp = bld.species(x, y, z, m, **skw)
+        torch.cuda.synchronize()
+        ts = (time.perf_counter() - t0) / args.species_steps
+        out["species"] = {"ms_per_call": 1000 * ts, "particles_per_s": n / ts, "npart_esp": esp.tolist(),
+                          "routine": "SORT_DM_PARTICLES_LOCALDENSITY (reader.f:140-336), NMAX = NX"}
+        if not args.no_cpu and n <= (1 << 25):
+            from oracle import oracle as O
+            hx = [a.cpu().numpy() if not isinstance(a, np.ndarray) else a for a in (x, y, z, m)]
+            t0 = time.perf_counter()
+            _, esp_o, _ = O.species32(*hx, g.nx, float(g.dx), rodo, re0)
+            tc = time.perf_counter() - t0
+            out["species"]["cpu_baseline"] = {"seconds": tc, "particles_per_s": n / tc, "cores": 1,
+                                              "kind": "port", "same_counts": bool(np.array_equal(esp_o, esp))}
+        log(f"[bench] species: {1000 * ts:.3f} ms/call")
+        del order
 
     # the forward workspaces (sized by the largest mode above) are not needed by
     # the adjoint; free them so the adjoint's own buffers fit at any size
