@@ -1230,7 +1230,7 @@ asohf_status asohf_hip_species(asohf_ctx* ctx, const int32_t nmax[3], const floa
   const size_t b4 = sizeof(unsigned) * (size_t)std::max<int64_t>(n, 1);
   for (int k = 0; k < 4; ++k) HIPCHK(ctx->sp_k[k].ensure(b4), "allocating the species sort");
   HIPCHK(ctx->sp_dens.ensure(sizeof(float) * (size_t)nc), "allocating the species grid");
-  HIPCHK(ctx->sp_cnt.ensure(sizeof(unsigned) * kNEsp), "allocating the species counts");
+  HIPCHK(ctx->sp_cnt.ensure(sizeof(unsigned) * (kNEsp + 2)), "allocating the species counts");
   size_t tb = 0;
   HIPCHK(species_sort_bytes((unsigned long long)std::max<int64_t>(n, 1), tb), "sizing the species sort");
   HIPCHK(ctx->sp_tmp.ensure(std::max<size_t>(tb, 1)), "allocating the species sort");

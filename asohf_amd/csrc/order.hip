@@ -50,18 +50,129 @@ __global__ void __launch_bounds__(256) species_keys(SpeciesGrid S, const float* 
   }
 }
 
+// Runs of equal keys in the sorted cell keys: the last position of each run
+// writes the run's end (exclusive) into runend[cell].  runend is the density
+// array itself, read back as unsigned by the run's first position before the
+// sums overwrite it (a cell's slot is touched by its own run only).
+__global__ void __launch_bounds__(256) species_run_ends(const unsigned* key, long long n, unsigned* runend) {
+  for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+       k += (long long)gridDim.x * blockDim.x) {
+    const unsigned c = key[k];
+    if (k + 1 == n || key[k + 1] != c) runend[c] = (unsigned)(k + 1);
+  }
+}
+
+constexpr unsigned kLongRun = 32;  // runs longer than this are summed by a whole wave
+
 // Cell densities: the masses of a cell summed in FP32 in particle order (the
 // pairs are sorted stably by cell), as the reference's loop does, then
-// divided by BAS.  One thread per cell, starting at its first sorted pair.
+// divided by BAS.  Runs of up to kLongRun particles: one thread each, loads
+// issued four at a time ahead of the (ordered) adds.  Longer runs are listed
+// as {start, end, cell} for species_sums_long.
 __global__ void __launch_bounds__(256) species_sums(SpeciesGrid S, const unsigned* key, const unsigned* idx,
-                                                    const float* m, long long n, float* dens) {
+                                                    const float* m, long long n, float* dens, uint4* longs,
+                                                    unsigned* nlong) {
   for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x) {
     const unsigned c = key[k];
     if (k > 0 && key[k - 1] == c) continue;
+    const unsigned b = (unsigned)k;
+    const unsigned e = reinterpret_cast<const unsigned*>(dens)[c];
+    if (e - b > kLongRun) {
+      longs[atomicAdd(nlong, 1u)] = make_uint4(b, e, c, 0u);
+      continue;
+    }
     float s = 0.f;
-    for (long long j = k; j < n && key[j] == c; ++j) s += m[idx[j]];
+    unsigned j = b;
+    for (; j + 4 <= e; j += 4) {
+      const unsigned i0 = idx[j], i1 = idx[j + 1], i2 = idx[j + 2], i3 = idx[j + 3];
+      const float a0 = m[i0], a1 = m[i1], a2 = m[i2], a3 = m[i3];
+      s += a0;
+      s += a1;
+      s += a2;
+      s += a3;
+    }
+    for (; j < e; ++j) s += m[idx[j]];
     dens[c] = s / S.bas;
+  }
+}
+
+// Long runs: one wave per run, taken from the list through a work counter.
+// The wave gathers the run's masses 1024 at a time (lane l holds positions
+// b + 64t + l, t = 0..15) with the next block's masses and the indices of the
+// block after it in flight, stages the block in its own LDS slice, and every
+// lane then adds the slice into one running sum in position order (broadcast
+// reads, four masses each), so the result is the sequential FP32 sum.  Lanes
+// past the run's end stage +0.0f, which leaves a sum of non-negative masses
+// unchanged.
+constexpr int kRunBlk = 16;  // 64-position groups per block
+constexpr int kSumWaves = 4;
+__global__ void __launch_bounds__(64 * kSumWaves) species_sums_long(SpeciesGrid S, const unsigned* idx,
+                                                                    const float* m, float* dens,
+                                                                    const uint4* longs, const unsigned* nlong,
+                                                                    unsigned* next) {
+  __shared__ float4 stage_s[kSumWaves][kRunBlk * 16];
+  const unsigned lane = threadIdx.x & 63u;
+  float4* stage = stage_s[threadIdx.x >> 6];
+  float* stage1 = reinterpret_cast<float*>(stage);
+  const unsigned total = *nlong;
+  for (;;) {
+    unsigned w = 0;
+    if (lane == 0) w = atomicAdd(next, 1u);
+    w = __builtin_amdgcn_readfirstlane(w);
+    if (w >= total) break;
+    const uint4 r = longs[w];
+    // positions in 64 bits: blocks ahead of the run's end stay past it
+    const unsigned long long b0 = r.x, e = r.y;
+    auto ld_idx = [&](unsigned long long b, unsigned (&o)[kRunBlk]) {
+#pragma unroll
+      for (int t = 0; t < kRunBlk; ++t) {
+        const unsigned long long p = b + 64u * t + lane;
+        o[t] = p < e ? idx[p] : 0xffffffffu;
+      }
+    };
+    auto ld_m = [&](const unsigned (&ix)[kRunBlk], float (&o)[kRunBlk]) {
+#pragma unroll
+      for (int t = 0; t < kRunBlk; ++t) o[t] = ix[t] != 0xffffffffu ? m[ix[t]] : 0.f;
+    };
+    unsigned ia[kRunBlk], ib[kRunBlk];
+    float ma[kRunBlk], mb[kRunBlk];
+    ld_idx(b0, ia);
+    ld_m(ia, ma);
+    ld_idx(b0 + 64u * kRunBlk, ib);
+    float s = 0.f;
+    for (unsigned long long b = b0; b < e; b += 64u * kRunBlk) {
+      ld_m(ib, mb);                          // masses of the next block
+      ld_idx(b + 2u * 64u * kRunBlk, ia);    // indices of the block after it
+#pragma unroll
+      for (int t = 0; t < kRunBlk; ++t) stage1[64 * t + lane] = ma[t];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int t = 0; t < kRunBlk; ++t) {
+        if (b + 64u * t >= e) break;
+        float4 v[16];  // the group's 64 masses read at once: one LDS latency per group
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = stage[16 * t + q];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          s += v[q].x;
+          s += v[q].y;
+          s += v[q].z;
+          s += v[q].w;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int t = 0; t < kRunBlk; ++t) {
+        ma[t] = mb[t];
+        ib[t] = ia[t];
+      }
+    }
+    if (lane == 0) dens[r.z] = s / S.bas;
   }
 }
 
@@ -98,7 +209,7 @@ __global__ void __launch_bounds__(256) widen_order(const unsigned* in, long long
 }
 
 __global__ void clear_counts(unsigned* counts) {
-  if (threadIdx.x < kNEsp) counts[threadIdx.x] = 0u;
+  if (threadIdx.x < kNEsp + 2) counts[threadIdx.x] = 0u;
 }
 
 static int species_grid(long long n, int cus) {
@@ -120,7 +231,7 @@ hipError_t species_sort_bytes(unsigned long long n, size_t& bytes) {
 }
 
 // The whole routine on the device: k[0..3] are four n-element u32 buffers,
-// counts holds kNEsp u32, dens nmax*nmay*nmaz floats, temp the sort scratch.
+// counts holds kNEsp + 2 u32, dens nmax*nmay*nmaz floats, temp the sort scratch.
 hipError_t launch_species(const int nmax[3], const float d[3], float rodo_re0_bas, long long n, const float* x,
                           const float* y, const float* z, const float* m, int* species, long long* order,
                           unsigned* counts, float* dens, unsigned* k0, unsigned* k1, unsigned* k2, unsigned* k3,
@@ -142,7 +253,13 @@ hipError_t launch_species(const int nmax[3], const float d[3], float rodo_re0_ba
   size_t tb = temp_bytes;
   hipError_t e = rocprim::radix_sort_pairs(temp, tb, k0, k2, k1, k3, (unsigned long long)n, 0u, (unsigned)bits, s);
   if (e != hipSuccess) return e;
-  species_sums<<<grid, 256, 0, s>>>(S, k2, k3, m, n, dens);
+  // k0 is free between the sort and species_assign: it holds the long-run list
+  // (at most n / kLongRun records of 4 words); counts[kNEsp] counts the list,
+  // counts[kNEsp + 1] hands its records out.
+  species_run_ends<<<grid, 256, 0, s>>>(k2, n, reinterpret_cast<unsigned*>(dens));
+  species_sums<<<grid, 256, 0, s>>>(S, k2, k3, m, n, dens, reinterpret_cast<uint4*>(k0), counts + kNEsp);
+  species_sums_long<<<cus * 4, 64 * kSumWaves, 0, s>>>(S, k3, m, dens, reinterpret_cast<const uint4*>(k0), counts + kNEsp,
+                                            counts + kNEsp + 1);
   species_assign<<<grid, 256, 0, s>>>(S, x, y, z, n, dens, species, k0, k1, counts);
   tb = temp_bytes;
   e = rocprim::radix_sort_pairs(temp, tb, k0, k2, k1, k3, (unsigned long long)n, 0u, 4u, s);

@@ -44,6 +44,74 @@ def edge_particles():
     return x, y.astype(np.float32), z, m, float(dx)
 
 
+def threshold_cells(big_first=True, seed=4):
+    """A cell whose FP32 density lands on either side of a class threshold
+    depending on the summation order: one heavy particle at 0.9999 of 8^3*BAS
+    and 10000 light ones, each below half an ulp of it.  Summed in particle
+    order with the heavy one first, the light masses round away (class 2);
+    summed in any order that adds light masses together first, the cell
+    crosses 8^3 (class 3).  Further cells hold runs of 31..2049 particles of
+    random masses (run lengths on both sides of the GPU's short/long and
+    block boundaries), interleaved with uniform particles in index order."""
+    rodo, re0 = consts()
+    rng = np.random.default_rng(seed)
+    L = np.float32(40.0)
+    dx = np.float32(L / 128)
+    bas = np.float32(np.float32(np.float32(dx * dx) * dx) * np.float32(rodo)) * np.float32(re0) ** 3
+    centre = lambda i, j, k: [np.float32((c + 0.5) * dx - L / 2) for c in (i, j, k)]
+    big = np.float32(512.0) * bas * np.float32(0.9999)
+    eps = np.float32(np.spacing(big) * np.float32(0.4))
+    xs, ms = [], []
+    cx = centre(100, 20, 77)
+    xs.append(np.tile(np.array(cx, np.float32), (10001, 1)))
+    cell = np.full(10001, eps, np.float32)
+    cell[0 if big_first else -1] = big
+    ms.append(cell)
+    for q, n_run in enumerate((31, 32, 33, 64, 1023, 1024, 1025, 2049)):
+        xs.append(np.tile(np.array(centre(10 + 9 * q, 90, 40), np.float32), (n_run, 1)))
+        ms.append((bas * np.float32(8.0) ** rng.uniform(0.5, 4.5) / np.float32(n_run)
+                   * rng.uniform(0.5, 1.5, n_run)).astype(np.float32))
+    xs.append(rng.uniform(-L / 2, L / 2, (5000, 3)).astype(np.float32))
+    ms.append(np.full(5000, bas * np.float32(0.3), np.float32))
+    pos = np.concatenate(xs)
+    m = np.concatenate(ms)
+    # interleave: random new indices, each cell keeping its own particles' order
+    perm = rng.permutation(len(m))
+    keys = [np.floor((pos[:, a].astype(np.float64) + L / 2) / dx).astype(np.int64) for a in range(3)]
+    flat = keys[0] + 128 * (keys[1] + 128 * keys[2])
+    new_index = np.empty(len(m), np.int64)
+    for c in np.unique(flat):
+        members = np.nonzero(flat == c)[0]                # original order
+        new_index[members] = np.sort(perm[members])       # that many random slots, ascending
+    out = np.empty((len(m), 3), np.float32)
+    mo = np.empty(len(m), np.float32)
+    out[new_index] = pos
+    mo[new_index] = m
+    return out[:, 0].copy(), out[:, 1].copy(), out[:, 2].copy(), mo, float(dx)
+
+
+def test_oracle_threshold_cell_is_order_sensitive():
+    """The threshold cell's class depends on the summation order, so the GPU
+    comparison on it checks the order of the FP32 sums."""
+    rodo, re0 = consts()
+    for big_first, want in ((True, 2), (False, 3)):
+        x, y, z, m, dx = threshold_cells(big_first)
+        sp, esp, order = O.species32(x, y, z, m, NMAX, dx, rodo, re0)
+        heavy = int(np.argmax(m))
+        assert sp[heavy] == want, (big_first, sp[heavy])
+        cell = (x == x[heavy]) & (y == y[heavy]) & (z == z[heavy])
+        assert cell.sum() == 10001 and np.all(sp[cell] == want)
+        # the light masses are interleaved with other particles in index order
+        idx = np.nonzero(cell)[0]
+        assert idx[-1] - idx[0] > 10001
+    # FP64 (any order) puts the cell in class 3: only the sequential FP32 sum gives 2
+    x, y, z, m, dx = threshold_cells(True)
+    heavy = int(np.argmax(m))
+    cell = (x == x[heavy]) & (y == y[heavy]) & (z == z[heavy])
+    bas = np.float64(np.float32(np.float32(np.float32(np.float32(dx * dx) * dx) * np.float32(rodo)) * np.float32(re0) ** 3))
+    assert np.floor(np.log(m[cell].astype(np.float64).sum() / bas) / np.log(8.0)) == 3
+
+
 @pytest.mark.skipif(not O.ref_available(), reason="the reference build (oracle/_ref) is not present")
 def test_oracle_matches_reference_routine():
     rodo, re0 = consts()
@@ -101,6 +169,8 @@ def test_gpu_species_matches_oracle():
     b = DensityBuilder(0)
     cases = [(name, s.x, s.y, s.z, s.m, float(s.geom.dx)) for name, s in snapshots()]
     cases.append(("edges",) + edge_particles())
+    cases.append(("threshold_cells",) + threshold_cells(True))
+    cases.append(("threshold_cells_reversed",) + threshold_cells(False))
     big = synth.make_snapshot(synth.CONFIGS[2])
     cases.append(("c2_full", big.x, big.y, big.z, big.m, float(big.geom.dx)))
     for name, x, y, z, m, dx in cases:
