@@ -17,6 +17,11 @@
 
 namespace asohf {
 
+// LDS copy of the kernel table rows 0..kNLevels: row irk holds 2^(irk+1) + 1
+// values and starts after the rows below it
+__host__ __device__ constexpr int ktab_off(int irk) { return (2 << irk) - 2 + irk; }
+constexpr int kKtabLds = ktab_off(kNLevels + 1);
+
 // ---------------------------------------------------------------------------
 // wave / block reductions
 // ---------------------------------------------------------------------------
@@ -651,11 +656,22 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
   __shared__ unsigned s_item;
   __shared__ long long s_redi[16];
   __shared__ double s_redd[16];
+  // table mode: the used part of the kernel table, row irk (2^(irk+1) + 1
+  // values) at ktab_off(irk), centred at ktab_off(irk) + 2^irk
+  __shared__ float s_ktab[MODE == 1 ? kKtabLds : 1];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const double dscale = fixed_scale(D.C->mmax_bits);
   const float scale = (float)dscale;
   const unsigned n_items = D.C->n_items_eff;
   const float k0[3] = {P.ktab[P.kmax - 1], P.ktab[P.kmax], P.ktab[P.kmax + 1]};
+  if (MODE == 1) {
+    for (int irk = 0; irk <= P.nl; ++irk) {
+      const int w = 2 * (1 << irk) + 1;
+      for (int j = tid; j < w; j += nthr)
+        s_ktab[ktab_off(irk) + j] = P.ktab[irk * P.kw + P.kmax - (1 << irk) + j];
+    }
+    // the first item's __syncthreads orders these writes before any read
+  }
   // interior cell (a, b, c) of the tile
   auto at = [](int a, int b, int c) { return (a + 2) + HX * ((b + 2) + HY * (c + 2)); };
 
@@ -741,27 +757,57 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
         vb = D.entries[min(e + 3 * nthr, last)];
       }
     } else if (MODE == 1) {
-      // ---- wide table-mode entries: one wave per entry, lanes over cells ----
+      // ---- wide table-mode entries: one wave per entry.  The entry's box
+      // clipped to the tile (bx x by x bz cells) is walked as one flat range,
+      // lane l taking cells l, l + 64, ...; its (x, y, z) counters advance by
+      // constant steps, so the loop has no divisions.  Kernel values come
+      // from the LDS copy of the table; the next entry is loaded while this
+      // one deposits.  Terms are ((m*Kx)*Ky)*Kz as in the reference's loop. ----
       const int wave = tid >> 6, lane = tid & 63, nwaves = nthr >> 6;
-      for (long long e = it.begin + wave; e < it.end; e += nwaves) {
+      long long e = it.begin + wave;
+      int4 nxt = e < it.end ? D.entries[e] : make_int4(0, 0, 0, 0);
+      for (; e < it.end; e += nwaves) {
         int ci[3], irk;
         float m;
-        unpack_table(D.entries[e], ci[0], ci[1], ci[2], irk, m);
+        unpack_table(nxt, ci[0], ci[1], ci[2], irk, m);
+        if (e + nwaves < it.end) nxt = D.entries[e + nwaves];
         const int nk = 1 << irk;
-        const float* K = P.ktab + irk * P.kw + P.kmax;
+        const float* K = s_ktab + ktab_off(irk) + nk;
         const int xa = max(max(ci[0] - nk, 1), i0), xb = min(min(ci[0] + nk, G.n[0]), i0 + ex - 1);
         const int ya = max(max(ci[1] - nk, 1), j0), yb = min(min(ci[1] + nk, G.n[1]), j0 + ey - 1);
         const int za = max(max(ci[2] - nk, 1), k0c), zb = min(min(ci[2] + nk, G.n[2]), k0c + ez - 1);
         if (xb < xa || yb < ya || zb < za) continue;
         const int bx = xb - xa + 1, by = yb - ya + 1, total = bx * by * (zb - za + 1);
+        // q / n for q <= 64 * 16 and n <= 16: (q + 1/2) / n is at least 1/32
+        // away from an integer, far beyond the FP32 error of the product
+        const float rbx = 1.0f / (float)bx, rby = 1.0f / (float)by;
+        auto divn = [](int q, float rn) { return (int)(((float)q + 0.5f) * rn); };
+        const int r0 = divn(lane, rbx);
+        int a = lane - r0 * bx;
+        int c = divn(r0, rby);
+        int b = r0 - c * by;
+        const int dr = divn(64, rbx), da = 64 - dr * bx;
+        const int dc0 = divn(dr, rby), db0 = dr - dc0 * by;
+        const int dc1 = divn(dr + 1, rby), db1 = dr + 1 - dc1 * by;
         const float ms = RED == 0 ? m * scale : m;
-        for (int c = lane; c < total; c += 64) {
-          const int a = c % bx, r = c / bx, b = r % by, cc = r / by;
-          const int ii = xa + a, jj = ya + b, kk = za + cc;
-          const float term = ((ms * K[ii - ci[0]]) * K[jj - ci[1]]) * K[kk - ci[2]];
-          Acc* dst = acc + at(ii - i0, jj - j0, kk - k0c);
+        const float* Kx = K + (xa - ci[0]);
+        const float* Ky = K + (ya - ci[1]);
+        const float* Kz = K + (za - ci[2]);
+        Acc* base = acc + at(xa - i0, ya - j0, za - k0c);
+        for (int q2 = lane; q2 < total; q2 += 64) {
+          const float term = ((ms * Kx[a]) * Ky[b]) * Kz[c];
+          Acc* dst = base + (a + HX * (b + HY * c));
           if (RED == 0) lds_add(reinterpret_cast<long long*>(dst), (long long)__float2int_rn(term));
           else lds_add(reinterpret_cast<float*>(dst), term);
+          a += da;
+          const bool carry = a >= bx;
+          if (carry) a -= bx;
+          b += carry ? db1 : db0;
+          c += carry ? dc1 : dc0;
+          if (b >= by) {
+            b -= by;
+            ++c;
+          }
         }
       }
     }
