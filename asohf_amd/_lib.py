@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libasohf_hip.so")
 
 ASOHF_OK = 0
+ASOHF_ERR_CAPACITY = -2
 STATUS = {0: "ASOHF_OK", -1: "ASOHF_ERR_INVALID", -2: "ASOHF_ERR_CAPACITY", -3: "ASOHF_ERR_HIP",
           -4: "ASOHF_ERR_OOM", -5: "ASOHF_ERR_NODEVICE"}
 MODE_TABLE, MODE_TSC = 1, 2
@@ -23,7 +24,8 @@ PATH_BATCHED, PATH_PLAIN = 0, 1
 EXPORTS = ("asohf_hip_create", "asohf_hip_destroy", "asohf_hip_last_error", "asohf_hip_density",
            "asohf_hip_density_host", "asohf_hip_density_adjoint", "asohf_hip_membership",
            "asohf_hip_get_tune", "asohf_hip_set_tune", "asohf_hip_autotune", "asohf_hip_get_stats",
-           "asohf_hip_release_workspaces", "asohf_hip_species", "asohf_hip_version")
+           "asohf_hip_release_workspaces", "asohf_hip_species", "asohf_hip_peak_candidates",
+           "asohf_hip_version")
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
@@ -103,6 +105,8 @@ def load():
         L.asohf_hip_release_workspaces.argtypes = [vp]
         L.asohf_hip_species.argtypes = [vp, _ip, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int64,
                                         vp, vp, vp, vp, vp, vp, _ip, vp]
+        L.asohf_hip_peak_candidates.argtypes = [vp, G, C.c_int32, vp, vp, vp, C.c_float, vp, vp, vp, vp, vp,
+                                                C.c_int64, C.POINTER(C.c_int64), O]
         for nm in EXPORTS:
             if nm not in ("asohf_hip_destroy", "asohf_hip_last_error", "asohf_hip_version"):
                 getattr(L, nm).restype = C.c_int32

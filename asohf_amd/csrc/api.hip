@@ -174,6 +174,7 @@ struct asohf_ctx {
   DBuf flags, idx, pool, pool_off, pool_nk, blk_pairs, blk_mmax, nsplit, item_off, split_off;
   DBuf elist, eoff, ecount;
   DBuf sp_k[4], sp_dens, sp_cnt, sp_tmp;   // species partition
+  DBuf pk_r[2], pk_k[2], pk_tmp, pk_cnt, pk_geo;   // peak candidates
   long long pool_want = 0;   // pairs the last count pass needed
   long long epool_want = 0;  // emission-list words the last count pass needed
   bool sized = false;        // the entry/item/scratch buffers were sized by an exact call
@@ -971,7 +972,9 @@ void asohf_hip_destroy(asohf_ctx* ctx) {
                   &ctx->scratch, &ctx->flags, &ctx->idx, &ctx->h_in[0], &ctx->h_in[1], &ctx->h_in[2],
                   &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass, &ctx->ord[0], &ctx->ord[1],
                   &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount, &ctx->sp_k[0],
-                  &ctx->sp_k[1], &ctx->sp_k[2], &ctx->sp_k[3], &ctx->sp_dens, &ctx->sp_cnt, &ctx->sp_tmp};
+                  &ctx->sp_k[1], &ctx->sp_k[2], &ctx->sp_k[3], &ctx->sp_dens, &ctx->sp_cnt, &ctx->sp_tmp,
+                  &ctx->pk_r[0], &ctx->pk_r[1], &ctx->pk_k[0], &ctx->pk_k[1], &ctx->pk_tmp, &ctx->pk_cnt,
+                  &ctx->pk_geo};
   for (DBuf* b : bufs) b->release();
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1246,6 +1249,112 @@ asohf_status asohf_hip_species(asohf_ctx* ctx, const int32_t nmax[3], const floa
   return ASOHF_OK;
 }
 
+asohf_status asohf_hip_peak_candidates(asohf_ctx* ctx, const asohf_geom* geom, int32_t level, const float* u1,
+                                       const float* u11, const int32_t* conta, float contrastec, float* dens,
+                                       int32_t* ix, int32_t* iy, int32_t* iz, int32_t* patch, int64_t cap,
+                                       int64_t* count, const asohf_opts* opts) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  if (!geom || !count || cap < 0 || (cap > 0 && (!dens || !ix || !iy || !iz)))
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL argument");
+  *count = 0;
+  if (geom->nx < 1 || geom->ny < 1 || geom->nz < 1 || geom->nl < 0 || geom->nl > kNLevels)
+    return fail(ctx, ASOHF_ERR_INVALID, "grid sizes or NL out of range");
+  if (level < 0 || level > geom->nl) return fail(ctx, ASOHF_ERR_INVALID, "level outside 0..NL");
+  if (geom->nl > 0 && (!geom->npatch || !geom->patchnx || !geom->patchny || !geom->patchnz))
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL patch arrays");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t s = opts ? (hipStream_t)opts->stream : (hipStream_t)0;
+  // CONTRASTECPEAK = CONTRASTEC/6.0 in FP32 (haloes_grids.f:288)
+  const float peak = contrastec / 6.0f;
+  long long ld1 = 0, ld2 = 0;
+  int np = 0, patch0 = 0;
+  std::vector<PeakPatchRec> recs;
+  if (level == 0) {
+    if (!u1) return fail(ctx, ASOHF_ERR_INVALID, "NULL u1");
+    const int l1 = (opts && opts->u1_ld[0] > 0) ? opts->u1_ld[0] : geom->nx;
+    const int l2 = (opts && opts->u1_ld[1] > 0) ? opts->u1_ld[1] : geom->ny;
+    if (l1 < geom->nx || l2 < geom->ny) return fail(ctx, ASOHF_ERR_INVALID, "u1 leading dimensions below NX, NY");
+    ld1 = l1;
+    ld2 = (long long)l1 * l2;
+  } else {
+    long long before = 0, total = 0;
+    for (int l = 1; l <= geom->nl; ++l) {
+      if (geom->npatch[l] < 0) return fail(ctx, ASOHF_ERR_INVALID, "negative NPATCH");
+      if (l < level) before += geom->npatch[l];
+      total += geom->npatch[l];
+    }
+    patch0 = (int)before;
+    np = geom->npatch[level];
+    const bool padded = opts && opts->u11_ld[0] > 0;
+    if (padded) {
+      ld1 = opts->u11_ld[0];
+      ld2 = (long long)opts->u11_ld[0] * opts->u11_ld[1];
+    }
+    long long off = 0, rank = 0;
+    for (long long p = 0; p < before + np; ++p) {
+      const int a = geom->patchnx[p], b = geom->patchny[p], c = geom->patchnz[p];
+      if (a < 1 || b < 1 || c < 1) return fail(ctx, ASOHF_ERR_INVALID, "patch sizes must be >= 1");
+      if (padded && (a > opts->u11_ld[0] || b > opts->u11_ld[1] || c > opts->u11_ld[2]))
+        return fail(ctx, ASOHF_ERR_INVALID, "a patch exceeds the padded U11 dimensions");
+      const long long cells = (long long)a * b * c;
+      const long long first = padded ? p * ld2 * opts->u11_ld[2] : off;
+      if (p >= before) {
+        PeakPatchRec r;
+        r.n[0] = a;
+        r.n[1] = b;
+        r.n[2] = c;
+        r.pad = 0;
+        r.off = first;
+        r.rank = rank;
+        recs.push_back(r);
+        rank += cells;
+      }
+      off += cells;
+    }
+    (void)total;
+    if (np > 0 && !u11) return fail(ctx, ASOHF_ERR_INVALID, "NULL u11");
+  }
+  const bool empty = level == 0 ? (geom->nx < 3 || geom->ny < 3 || geom->nz < 3) : np == 0;
+  if (empty) return ASOHF_OK;
+  if (np > 0) {
+    HIPCHK(ctx->pk_geo.ensure(sizeof(PeakPatchRec) * recs.size()), "allocating the patch records");
+    HIPCHK(hipMemcpyAsync(ctx->pk_geo.p, recs.data(), sizeof(PeakPatchRec) * recs.size(), hipMemcpyHostToDevice, s),
+           "uploading the patch records");
+  }
+  HIPCHK(ctx->pk_cnt.ensure(sizeof(unsigned long long)), "allocating the candidate counter");
+  const PeakPatchRec* dpp = np > 0 ? ctx->pk_geo.as<PeakPatchRec>() : nullptr;
+  unsigned long long want = (unsigned long long)std::max<int64_t>(cap, 1 << 16), k = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int b = 0; b < 2; ++b) {
+      HIPCHK(ctx->pk_r[b].ensure(sizeof(unsigned long long) * want), "allocating the candidates");
+      HIPCHK(ctx->pk_k[b].ensure(sizeof(unsigned) * want), "allocating the candidates");
+    }
+    const unsigned long long have = std::min(ctx->pk_r[0].cap / sizeof(unsigned long long),
+                                             ctx->pk_k[0].cap / sizeof(unsigned));
+    HIPCHK(launch_peak_scan(level, level == 0 ? u1 : u11, conta, geom->nx, geom->ny, geom->nz, ld1, ld2, dpp, np,
+                            peak, ctx->pk_r[0].as<unsigned long long>(), ctx->pk_k[0].as<unsigned>(), have,
+                            ctx->pk_cnt.as<unsigned long long>(), ctx->cus, s),
+           "scanning for peak candidates");
+    HIPCHK(hipMemcpyAsync(&k, ctx->pk_cnt.p, sizeof(k), hipMemcpyDeviceToHost, s), "reading the count");
+    HIPCHK(hipStreamSynchronize(s), "peak scan");
+    if (k <= have) break;
+    want = k;   // every candidate did not fit: grow and scan again
+  }
+  *count = (int64_t)k;
+  if ((int64_t)k > cap) return fail(ctx, ASOHF_ERR_CAPACITY, "more peak candidates than cap (count set)");
+  size_t tb = 0;
+  HIPCHK(peak_sort_bytes(std::max<unsigned long long>(k, 1), tb), "sizing the candidate sort");
+  HIPCHK(ctx->pk_tmp.ensure(std::max<size_t>(tb, 1)), "allocating the candidate sort");
+  HIPCHK(launch_peak_sort_write(ctx->pk_r[0].as<unsigned long long>(), ctx->pk_k[0].as<unsigned>(),
+                                ctx->pk_r[1].as<unsigned long long>(), ctx->pk_k[1].as<unsigned>(), (long long)k,
+                                ctx->pk_tmp.p, ctx->pk_tmp.cap, level, geom->nx, geom->ny, dpp, np, patch0, dens,
+                                ix, iy, iz, patch, ctx->cus, s),
+         "sorting the peak candidates");
+  HIPCHK(hipStreamSynchronize(s), "peak candidates");
+  return ASOHF_OK;
+}
+
 asohf_status asohf_hip_release_workspaces(asohf_ctx* ctx) {
   if (!ctx) return ASOHF_ERR_INVALID;
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
@@ -1259,7 +1368,8 @@ asohf_status asohf_hip_release_workspaces(asohf_ctx* ctx) {
                   &ctx->h_in[2], &ctx->h_in[3], &ctx->h_out[0], &ctx->h_out[1], &ctx->h_mass, &ctx->ord[0],
                   &ctx->ord[1], &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount,
                   &ctx->sp_k[0], &ctx->sp_k[1], &ctx->sp_k[2], &ctx->sp_k[3], &ctx->sp_dens, &ctx->sp_cnt,
-                  &ctx->sp_tmp};
+                  &ctx->sp_tmp, &ctx->pk_r[0], &ctx->pk_r[1], &ctx->pk_k[0], &ctx->pk_k[1], &ctx->pk_tmp,
+                  &ctx->pk_cnt, &ctx->pk_geo};
   for (DBuf* b : bufs) b->release();
   ctx->sized = false;   // the next batched call sizes its buffers again
   return ASOHF_OK;

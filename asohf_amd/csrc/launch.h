@@ -44,6 +44,22 @@ hipError_t launch_species(const int nmax[3], const float d[3], float bas, long l
 hipError_t sort_cell_pairs(void* temp, size_t& temp_bytes, const unsigned* key_in, unsigned* key_out,
                            const unsigned* idx_in, unsigned* idx_out, unsigned long long n, int end_bit,
                            hipStream_t s);
+// Peak candidates of the grid halo finder (peaks.hip)
+struct PeakPatchRec {
+  int n[3];
+  int pad;
+  long long off;
+  long long rank;
+};
+hipError_t peak_sort_bytes(unsigned long long n, size_t& bytes);
+hipError_t launch_peak_scan(int level, const float* u, const int* conta, int nx, int ny, int nz, long long ld1,
+                            long long ld2, const PeakPatchRec* pp, int np, float peak, unsigned long long* rec_r,
+                            unsigned* rec_k, unsigned long long cap, unsigned long long* count, int cus,
+                            hipStream_t s);
+hipError_t launch_peak_sort_write(unsigned long long* rec_r, unsigned* rec_k, unsigned long long* rec_r2,
+                                  unsigned* rec_k2, long long k, void* temp, size_t temp_bytes, int level, int nx,
+                                  int ny, const PeakPatchRec* pp, int np, int patch0, float* dens, int* ix,
+                                  int* iy, int* iz, int* patch, int cus, hipStream_t s);
 hipError_t launch_plain_zero(const Plan& P, float* u1, float* u11, hipStream_t s);
 hipError_t launch_plain_base(const Plan& P, const Parts& A, float* u1, int cus, hipStream_t s);
 hipError_t launch_plain_flags(const Plan& P, const Parts& A, int l, unsigned* flag, int cus, hipStream_t s);

@@ -108,6 +108,36 @@ class DensityBuilder:
         _lib.check(self._h, st)
         return (sp[:n] if sp is not None else None), order[:n], esp
 
+    def peak_candidates(self, geom: Geometry, level: int, u1=None, u11=None, conta=None, *,
+                        contrastec: float, u1_ld=None, u11_ld=None, cap=None):
+        """Peak candidates of the grid halo finder on one level
+        (HALOFIND_GRID, haloes_grids.f:316-404 / 700-790), sorted as
+        SORT_CELLS_AMR (haloes_grids.f:1383-1425): cells at or above
+        CONTRASTEC/6 that exceed their six face neighbours, away from the
+        grid border, with CONTA1 (conta, int32, U11's layout) equal to 1 on
+        AMR levels.  Returns (dens, ix, iy, iz, patch) device tensors; patch
+        is the 1-based global patch number, 0 on the base grid."""
+        dev = (u1 if level == 0 else u11).device
+        if cap is None:
+            cap = 1 << 16
+        o = self._opts("tsc", u1_ld=u1_ld, u11_ld=u11_ld)
+        ga = _GeomArgs(geom)
+        L = _lib.load()
+        while True:
+            outs = [torch.empty(max(cap, 1), dtype=torch.float32, device=dev)] + \
+                   [torch.empty(max(cap, 1), dtype=torch.int32, device=dev) for _ in range(4)]
+            cnt = C.c_int64()
+            st = L.asohf_hip_peak_candidates(self._h, ga.ref, int(level),
+                                             _ptr(u1) if u1 is not None else None,
+                                             _ptr(u11) if u11 is not None else None,
+                                             _ptr(conta) if conta is not None else None, C.c_float(contrastec),
+                                             *[_ptr(t) for t in outs], cap, C.byref(cnt), C.byref(o))
+            if st == _lib.ASOHF_ERR_CAPACITY and cnt.value > cap:
+                cap = cnt.value
+                continue
+            _lib.check(self._h, st)
+            return tuple(t[: cnt.value] for t in outs)
+
     def release_workspaces(self):
         """Free the device workspaces kept between calls (the plan stays)."""
         _lib.check(self._h, _lib.load().asohf_hip_release_workspaces(self._h))

@@ -20,6 +20,8 @@ Prints ONE JSON line on stdout; progress goes to stderr.
              (deterministic, atomic) on the same snapshot
   per_level_ms  builds of the hierarchy cut at level L = 0..NL, differenced
   species    the species partition that feeds NPART_ESP (reader.f:140-336)
+  peaks      the grid halo finder's peak candidates on every level of the
+             field (haloes_grids.f:316-404, 700-790, SORT_CELLS_AMR order)
   adjoint    the hand-written backward (gradients of sum(w * rho))
   plain_gpu  the same build as one launch per grid with global FP32 atomics
   cpu_baseline  the oracle's OpenMP restatement of the reference algorithm
@@ -66,6 +68,9 @@ def main() -> int:
     ap.add_argument("--modes-steps", type=int, default=5, help="steps per mode x reduction (0: skip)")
     ap.add_argument("--no-levels", dest="levels", action="store_false", help="skip the per-level timing")
     ap.add_argument("--species-steps", type=int, default=5, help="calls of the species partition (0: skip)")
+    ap.add_argument("--peaks-steps", type=int, default=5,
+                    help="calls of the peak-candidate scan over every level (0: skip)")
+    ap.add_argument("--contrastec", type=float, default=200.0, help="CONTRASTEC of the peak-candidate scan")
     ap.add_argument("--plain-max-tests", type=float, default=5e12,
                     help="skip the plain form above this many particle-box tests")
     ap.add_argument("--tune", default="", help="tx,ty,tz,chunk,waves_per_cu[,block]: fixed tuning, no autotune")
@@ -250,6 +255,47 @@ def main() -> int:
                                               "kind": "port", "same_counts": bool(np.array_equal(esp_o, esp))}
         log(f"[bench] species: {1000 * ts:.3f} ms/call")
         del order
+
+    if args.peaks_steps > 0:
+        # the grid halo finder's peak candidates on every level of the field
+        # just built (HALOFIND_GRID, haloes_grids.f:316-404 and 700-790, in
+        # SORT_CELLS_AMR order); one call per level
+        r = bld.density(g, x, y, z, m, snap.npart_esp, **kw)
+
+        def scan_all():
+            return [bld.peak_candidates(g, lev, u1=r.u1 if lev == 0 else None, u11=r.u11 if lev else None,
+                                        contrastec=args.contrastec) for lev in range(g.nl + 1)]
+        res = scan_all()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.peaks_steps):
+            res = scan_all()
+        torch.cuda.synchronize()
+        tp = (time.perf_counter() - t0) / args.peaks_steps
+        counts = [int(c[0].numel()) for c in res]
+        out["peaks"] = {"ms_per_call_all_levels": 1000 * tp, "cells_per_s": n_cells / tp,
+                        "candidates_per_level": counts, "contrastec": args.contrastec,
+                        "routine": "HALOFIND_GRID peak candidates (haloes_grids.f:316-404, 700-790) "
+                                   "+ SORT_CELLS_AMR (haloes_grids.f:1383-1425)"}
+        if not args.no_cpu and n_cells <= (1 << 26):
+            from oracle import oracle as O
+            hu1 = r.u1.cpu().numpy().reshape(g.nz, g.ny, g.nx)
+            hu11 = r.u11.cpu().numpy()
+            offs = np.concatenate([[0], np.cumsum(g.patch_cells())])
+            grids, q = [[hu1]], 0
+            for lev in range(1, g.nl + 1):
+                grids.append([hu11[offs[p]:offs[p + 1]].reshape(g.pnz[p], g.pny[p], g.pnx[p])
+                              for p in range(q, q + int(g.npatch[lev]))])
+                q += int(g.npatch[lev])
+            t0 = time.perf_counter()
+            cpu = [O.peaks32(fl, args.contrastec) if fl else (np.zeros(0),) for fl in grids]
+            tc = time.perf_counter() - t0
+            same = all(np.array_equal(c[0], gp[0].cpu().numpy()) and
+                       (len(c) == 1 or np.array_equal(c[1], gp[1].cpu().numpy())) for c, gp in zip(cpu, res))
+            out["peaks"]["cpu_baseline"] = {"seconds": tc, "cells_per_s": n_cells / tc, "cores": 1,
+                                            "kind": "port", "same_lists": bool(same)}
+        log(f"[bench] peaks: {1000 * tp:.3f} ms/call over {g.nl + 1} levels, candidates {counts}")
+        del res
 
     # the forward workspaces (sized by the largest mode above) are not needed by
     # the adjoint; free them so the adjoint's own buffers fit at any size

@@ -185,6 +185,29 @@ asohf_status asohf_hip_species(asohf_ctx *ctx, const int32_t nmax[3], const floa
                                const float *m, int32_t *species, int64_t *order, int32_t *npart_esp,
                                void *stream);
 
+/* Peak candidates of the grid halo finder: the cells HALOFIND_GRID starts
+ * its haloes from (haloes_grids.f:316-404 on the base grid, 700-790 on AMR
+ * level IR), in the order SORT_CELLS / SORT_CELLS_AMR (haloes_grids.f:
+ * 1341-1425, with ARGSORT num.f:150-200) gives them.  A cell outside the
+ * one-cell border of its grid is a candidate when its density is at least
+ * CONTRASTEC/6 (FP32), its CONTA1 mask is 1 (AMR levels) and it exceeds its
+ * six face neighbours.  Order: 1/density ascending; equal keys by
+ * decreasing position in the reference's scan order (grid, z, y, x).
+ *   level 0  : u1 (device FP32, leading dims opts->u1_ld as asohf_hip_density)
+ *   level IR : u11 (device FP32, packed or padded as opts->u11_ld) for the
+ *              patches of level IR; conta (device int32, same layout) or NULL
+ *              (all ones)
+ * Outputs (device, cap entries): dens = 1/(1/density) as the reference
+ * writes DDD back, ix/iy/iz 1-based cell indices (DDDX..Z), patch (DDDP: the
+ * 1-based global patch number, 0 on the base grid; may be NULL).  *count
+ * receives the number of candidates; ASOHF_ERR_CAPACITY if it exceeds cap
+ * (nothing written).  Runs on opts->stream and synchronises before
+ * returning. */
+asohf_status asohf_hip_peak_candidates(asohf_ctx *ctx, const asohf_geom *geom, int32_t level, const float *u1,
+                                       const float *u11, const int32_t *conta, float contrastec, float *dens,
+                                       int32_t *ix, int32_t *iy, int32_t *iz, int32_t *patch, int64_t cap,
+                                       int64_t *count, const asohf_opts *opts);
+
 /* Frees the device workspaces the context keeps between calls (binned
  * entries, work items, block tables, emission lists, split tiles, the
  * adjoint's sort buffers, the captured graph); the geometry plan stays, and

@@ -12,7 +12,13 @@
  *   - INTERPOLATE_DENSITY_TSC  ("mode B") grids.f:1070-1398 (orc_density_b32 / _b64)
  *   - RENORM_DENSITY           grids.f:2668-2722  (the final "/ fdm" of every cell)
  * plus an FP64 adjoint of mode B (orc_adjoint_b64), which has no reference
- * counterpart (the reference has no derivatives).
+ * counterpart (the reference has no derivatives), and two routines beside
+ * the path:
+ *   - SORT_DM_PARTICLES_LOCALDENSITY reader.f:140-336 (orc_species32)
+ *   - the peak candidates of HALOFIND_GRID, haloes_grids.f:316-404 (base)
+ *     and 700-790 (AMR levels), sorted by SORT_CELLS / SORT_CELLS_AMR
+ *     (haloes_grids.f:1341-1425) with ARGSORT (num.f:150-200)
+ *     (orc_argsort32, orc_peaks32).
  *
  * Two arithmetic flavours:
  *   *32  -- the reference's own FP32 expressions, operand order and loop
@@ -1089,4 +1095,101 @@ int orc_species32(int nx, int ny, int nz, float dx, float dy, float dz, float ro
     npart_esp[s] = (int32_t)(k - c0);
   }
   return 0;
+}
+
+/* ARGSORT (num.f:150-200): bottom-up merge sort of indices 1..n by a[] in
+ * increasing order.  Runs of width DELTA start at 1, 1+2*DELTA, ...; a merge
+ * takes the left head only when it is strictly smaller, so of two equal keys
+ * the one from the right run goes first.  indx is 1-based as in Fortran. */
+void orc_argsort32(int64_t n, const float* a, int32_t* indx) {
+  for (int64_t i = 0; i < n; ++i) indx[i] = (int32_t)(i + 1);
+  if (n <= 1) return;
+  int32_t* tmp = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  for (int64_t delta = 1; delta < n; delta *= 2) {
+    for (int64_t iii = 1; iii <= n - delta; iii += 2 * delta) {
+      int64_t i1 = iii, i2 = iii + delta;
+      const int64_t kkk = (2 * delta < n - iii + 1) ? 2 * delta : n - iii + 1;
+      int64_t k = 1;
+      while (i1 < iii + delta && i2 < iii + kkk) {
+        if (a[indx[i1 - 1] - 1] < a[indx[i2 - 1] - 1]) tmp[(k++) - 1] = indx[(i1++) - 1];
+        else tmp[(k++) - 1] = indx[(i2++) - 1];
+      }
+      if (i1 < iii + delta) {
+        for (; k <= kkk; ++k) tmp[k - 1] = indx[(i1++) - 1];
+      } else {
+        for (; k <= kkk; ++k) tmp[k - 1] = indx[(i2++) - 1];
+      }
+      for (int64_t q = 0; q < kkk; ++q) indx[iii - 1 + q] = tmp[q];
+    }
+  }
+  free(tmp);
+}
+
+/* Peak candidates of HALOFIND_GRID on a set of grids g = 0..ng-1 (the base
+ * grid alone, or the patches of one level in patch order): field f, x
+ * fastest, grid g at off[g] with n1[g] x n2[g] x n3[g] cells; mask (same
+ * layout, or NULL for all ones) plays CONTA1.  A cell outside the one-cell
+ * border (BOR = BORAMR = 1) is a candidate when f >= CONTRASTEC/6.0, its
+ * mask is 1, and it is a strict local maximum along the six face
+ * neighbours, tested as the reference does (haloes_grids.f:325-347 and
+ * 712-734: f(x+1)-f(x) < 0 and f(x)-f(x-1) > 0, likewise in y and z).
+ * Candidates are collected in grid, z, y, x order and sorted as SORT_CELLS_AMR
+ * does: keys 1/f, ARGSORT, and f returned as 1/(1/f).  Returns the number of
+ * candidates; the first min(K, cap) are written (ix, iy, iz 1-based). */
+int64_t orc_peaks32(int ng, const int32_t* n1, const int32_t* n2, const int32_t* n3, const int64_t* off,
+                    const float* f, const int32_t* mask, float contrastec, int64_t cap, float* dens,
+                    int32_t* ix, int32_t* iy, int32_t* iz, int32_t* grid) {
+  const float peak = contrastec / 6.0f;
+  int64_t k = 0, kcap = 1024;
+  float* d = (float*)malloc(sizeof(float) * (size_t)kcap);
+  int32_t* cx = (int32_t*)malloc(sizeof(int32_t) * (size_t)kcap * 4);
+  for (int g = 0; g < ng; ++g) {
+    const int a = n1[g], b = n2[g], c = n3[g];
+    const float* u = f + off[g];
+    const int32_t* mk = mask ? mask + off[g] : NULL;
+#define U(i, j, l) u[(size_t)((i) - 1) + (size_t)a * ((size_t)((j) - 1) + (size_t)b * (size_t)((l) - 1))]
+    for (int kz = 2; kz <= c - 1; ++kz)
+      for (int jy = 2; jy <= b - 1; ++jy)
+        for (int ii = 2; ii <= a - 1; ++ii) {
+          const float bas = U(ii, jy, kz);
+          if (!(bas >= peak)) continue;
+          if (mk && mk[(size_t)(ii - 1) + (size_t)a * ((size_t)(jy - 1) + (size_t)b * (size_t)(kz - 1))] != 1)
+            continue;
+          if (!(U(ii + 1, jy, kz) - bas < 0.0f)) continue;
+          if (!(U(ii, jy + 1, kz) - bas < 0.0f)) continue;
+          if (!(U(ii, jy, kz + 1) - bas < 0.0f)) continue;
+          if (!(bas - U(ii - 1, jy, kz) > 0.0f)) continue;
+          if (!(bas - U(ii, jy - 1, kz) > 0.0f)) continue;
+          if (!(bas - U(ii, jy, kz - 1) > 0.0f)) continue;
+          if (k == kcap) {
+            kcap *= 2;
+            d = (float*)realloc(d, sizeof(float) * (size_t)kcap);
+            cx = (int32_t*)realloc(cx, sizeof(int32_t) * (size_t)kcap * 4);
+          }
+          d[k] = bas;
+          cx[4 * k] = ii;
+          cx[4 * k + 1] = jy;
+          cx[4 * k + 2] = kz;
+          cx[4 * k + 3] = g;
+          ++k;
+        }
+#undef U
+  }
+  float* d2 = (float*)malloc(sizeof(float) * (size_t)(k > 0 ? k : 1));
+  int32_t* ind = (int32_t*)malloc(sizeof(int32_t) * (size_t)(k > 0 ? k : 1));
+  for (int64_t i = 0; i < k; ++i) d2[i] = 1.0f / d[i];
+  orc_argsort32(k, d2, ind);
+  for (int64_t i = 0; i < k && i < cap; ++i) {
+    const int64_t j = ind[i] - 1;
+    dens[i] = 1.0f / d2[j];
+    ix[i] = cx[4 * j];
+    iy[i] = cx[4 * j + 1];
+    iz[i] = cx[4 * j + 2];
+    grid[i] = cx[4 * j + 3];
+  }
+  free(d);
+  free(cx);
+  free(d2);
+  free(ind);
+  return k;
 }

@@ -7,7 +7,9 @@ module, and only as the checker or the timed CPU baseline.  The product path
   - the C restatement in asohf_oracle.c (libasohf_oracle.so), FP32 and FP64;
   - run_ref(): the reference's own Fortran DENSITY / INTERPOLATE_DENSITY_TSC
     compiled from the reference sources into _ref/asohf_ref_density
-    (built by `make ref` where the reference tree exists).
+    (built by `make ref` where the reference tree exists); run_ref_species()
+    and run_ref_sort_cells() call SORT_DM_PARTICLES_LOCALDENSITY and
+    SORT_CELLS_AMR through the same driver.
 """
 from __future__ import annotations
 
@@ -306,3 +308,67 @@ def run_ref(snap, mode: str = "A"):
     u1 = raw[o:o + nc].reshape(geom.nz, geom.ny, geom.nx)
     u11 = raw[o + nc:]
     return d[0], d[1], d[2], radx, u1, u11
+
+
+def argsort32(a):
+    """ARGSORT (num.f:150-200) restated: 1-based indices sorting a (FP32)
+    increasingly, the reference's merge order on ties."""
+    a = np.ascontiguousarray(a, np.float32)
+    out = np.empty(len(a), np.int32)
+    L = lib()
+    L.orc_argsort32.restype = None
+    L.orc_argsort32(C.c_int64(len(a)), _p(a, _fp), _p(out, _ip))
+    return out
+
+
+def peaks32(fields, contrastec: float, masks=None):
+    """Peak candidates of HALOFIND_GRID (haloes_grids.f:316-404, 700-790) on
+    a list of grids (3-D FP32 arrays indexed [z, y, x]; the base grid alone or
+    one level's patches in patch order), with optional CONTA1 masks (int32,
+    same shapes), sorted as SORT_CELLS_AMR (haloes_grids.f:1383-1425).
+    Returns (dens, ix, iy, iz, grid): grid is the position in `fields`;
+    ix, iy, iz are 1-based."""
+    shapes = [np.asarray(f).shape for f in fields]
+    n1 = np.array([sh[2] for sh in shapes], np.int32)
+    n2 = np.array([sh[1] for sh in shapes], np.int32)
+    n3 = np.array([sh[0] for sh in shapes], np.int32)
+    sizes = n1.astype(np.int64) * n2 * n3
+    off = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    flat = np.concatenate([np.ascontiguousarray(f, np.float32).ravel() for f in fields]) if fields else np.zeros(1, np.float32)
+    mk = None
+    if masks is not None:
+        mk = np.concatenate([np.ascontiguousarray(q, np.int32).ravel() for q in masks])
+    cap = max(int(sizes.sum()) // 8 + 16, 16)
+    L = lib()
+    L.orc_peaks32.restype = C.c_int64
+    while True:
+        outs = [np.empty(cap, np.float32)] + [np.empty(cap, np.int32) for _ in range(4)]
+        k = L.orc_peaks32(C.c_int(len(fields)), _p(n1, _ip), _p(n2, _ip), _p(n3, _ip),
+                          _p(off, C.POINTER(C.c_int64)), _p(flat, _fp),
+                          _p(mk, _ip) if mk is not None else None, C.c_float(contrastec),
+                          C.c_int64(cap), _p(outs[0], _fp), *[_p(o, _ip) for o in outs[1:]])
+        if k <= cap:
+            return tuple(o[:k] for o in outs)
+        cap = int(k)
+
+
+def run_ref_sort_cells(ddd, dddx, dddy, dddz, dddp):
+    """The reference's SORT_CELLS_AMR (haloes_grids.f:1383-1425) on the given
+    candidate arrays: returns the five arrays after its sort."""
+    ddd = np.ascontiguousarray(ddd, np.float32)
+    ints = [np.ascontiguousarray(a, np.int32) for a in (dddx, dddy, dddz, dddp)]
+    k = len(ddd)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([4, 0, 0, 0, 0, k, 0, 0], np.int32).tofile(f)
+            ddd.tofile(f)
+            for a in ints:
+                a.tofile(f)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([REF_BIN, fin, fout], check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, env=env)
+        raw = np.fromfile(fout, np.uint8)
+    d = np.frombuffer(raw[:4 * k].tobytes(), np.float32).copy()
+    rest = [np.frombuffer(raw[4 * k * (i + 1):4 * k * (i + 2)].tobytes(), np.int32).copy() for i in range(4)]
+    return (d, *rest)

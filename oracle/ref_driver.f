@@ -10,6 +10,10 @@
 *               with RODO = SCAL(2) and RE0 = SCAL(3); writes DX, DY, DZ,
 *               NPART_ESP(0:8) and ORIPA(1:N) (the original index of the
 *               particle at each position after the partition)
+*       mode 4: SUBROUTINE SORT_CELLS_AMR (haloes_grids.f:1383-1425) on
+*               K = hdr(6) candidate cells: input real*4 ddd(K), int32
+*               dddx(K), dddy(K), dddz(K), dddp(K) after the header;
+*               output the same five arrays after the sort
 *     The grid constants come from the reference's MALLA (grids.f:2509)
 *     called with LADO = LADO0 - LADO0/NX as asohf.f:323 does.
 *
@@ -49,6 +53,8 @@
       INTEGER NPART_ESP(0:N_ESP-1)
       REAL*4 SCAL(4),LADO0,LADO,FDM,T,ZETA
       CHARACTER*512 FIN,FOUT
+      REAL*4, ALLOCATABLE :: DDD(:)
+      INTEGER, ALLOCATABLE :: DDDX(:),DDDY(:),DDDZ(:),DDDP(:)
 
       CALL GET_COMMAND_ARGUMENT(1,FIN)
       CALL GET_COMMAND_ARGUMENT(2,FOUT)
@@ -56,6 +62,26 @@
      &     STATUS='OLD',ACTION='READ')
       READ(10) HDR
       MODE=HDR(1)
+      IF (MODE.EQ.4) THEN
+       N=HDR(6)
+       ALLOCATE(DDD(N),DDDX(N),DDDY(N),DDDZ(N),DDDP(N))
+       READ(10) (DDD(I),I=1,N)
+       READ(10) (DDDX(I),I=1,N)
+       READ(10) (DDDY(I),I=1,N)
+       READ(10) (DDDZ(I),I=1,N)
+       READ(10) (DDDP(I),I=1,N)
+       CLOSE(10)
+       CALL SORT_CELLS_AMR(N,DDD,DDDX,DDDY,DDDZ,DDDP)
+       OPEN(11,FILE=FOUT,ACCESS='STREAM',FORM='UNFORMATTED',
+     &      STATUS='REPLACE',ACTION='WRITE')
+       WRITE(11) (DDD(I),I=1,N)
+       WRITE(11) (DDDX(I),I=1,N)
+       WRITE(11) (DDDY(I),I=1,N)
+       WRITE(11) (DDDZ(I),I=1,N)
+       WRITE(11) (DDDP(I),I=1,N)
+       CLOSE(11)
+       STOP
+      END IF
       NX=HDR(2)
       NY=HDR(3)
       NZ=HDR(4)
