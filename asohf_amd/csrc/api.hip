@@ -175,6 +175,7 @@ struct asohf_ctx {
   DBuf elist, eoff, ecount;
   DBuf sp_k[4], sp_dens, sp_cnt, sp_tmp;   // species partition
   DBuf pk_r[2], pk_k[2], pk_tmp, pk_cnt, pk_geo;   // peak candidates
+  DBuf mk_geo, mk_err, vg_cnt, vg_start, vg_pairs;  // grid masks
   long long pool_want = 0;   // pairs the last count pass needed
   long long epool_want = 0;  // emission-list words the last count pass needed
   bool sized = false;        // the entry/item/scratch buffers were sized by an exact call
@@ -974,7 +975,7 @@ void asohf_hip_destroy(asohf_ctx* ctx) {
                   &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount, &ctx->sp_k[0],
                   &ctx->sp_k[1], &ctx->sp_k[2], &ctx->sp_k[3], &ctx->sp_dens, &ctx->sp_cnt, &ctx->sp_tmp,
                   &ctx->pk_r[0], &ctx->pk_r[1], &ctx->pk_k[0], &ctx->pk_k[1], &ctx->pk_tmp, &ctx->pk_cnt,
-                  &ctx->pk_geo};
+                  &ctx->pk_geo, &ctx->mk_geo, &ctx->mk_err, &ctx->vg_cnt, &ctx->vg_start, &ctx->vg_pairs};
   for (DBuf* b : bufs) b->release();
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1355,6 +1356,334 @@ asohf_status asohf_hip_peak_candidates(asohf_ctx* ctx, const asohf_geom* geom, i
   return ASOHF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Grid masks and the density dump (asohf.f:515-545)
+// ---------------------------------------------------------------------------
+// The patch records of every level, offsets in the U11 layout of opts.
+static asohf_status mask_patches(asohf_ctx* ctx, const asohf_geom* g, const int32_t* px, const int32_t* py,
+                                 const int32_t* pz, const int32_t* pare, const asohf_opts* opts,
+                                 std::vector<MaskPatch>& mp, std::vector<int>& lev_lo, std::vector<int>& lev_n,
+                                 long long& s1p, long long& s2p, long long& cells11) {
+  if (g->nx < 1 || g->ny < 1 || g->nz < 1 || g->nl < 0 || g->nl > kNLevels)
+    return fail(ctx, ASOHF_ERR_INVALID, "grid sizes or NL out of range");
+  if (g->nl > 0 && (!g->npatch || !g->patchnx || !g->patchny || !g->patchnz || !g->patchrx || !g->patchry ||
+                    !g->patchrz))
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL patch arrays");
+  lev_lo.assign(g->nl + 2, 0);
+  lev_n.assign(g->nl + 2, 0);
+  long long P = 0;
+  for (int l = 1; l <= g->nl; ++l) {
+    if (g->npatch[l] < 0) return fail(ctx, ASOHF_ERR_INVALID, "negative NPATCH");
+    lev_lo[l] = (int)P;
+    lev_n[l] = g->npatch[l];
+    P += g->npatch[l];
+  }
+  if (P > (1ll << 30)) return fail(ctx, ASOHF_ERR_CAPACITY, "too many patches");
+  const bool padded = opts && opts->u11_ld[0] > 0;
+  s1p = padded ? opts->u11_ld[0] : 0;
+  s2p = padded ? (long long)opts->u11_ld[0] * opts->u11_ld[1] : 0;
+  const long long slab = padded ? s2p * opts->u11_ld[2] : 0;
+  mp.resize((size_t)P);
+  long long off = 0;
+  for (long long p = 0; p < P; ++p) {
+    MaskPatch& m = mp[(size_t)p];
+    m.n[0] = g->patchnx[p];
+    m.n[1] = g->patchny[p];
+    m.n[2] = g->patchnz[p];
+    if (m.n[0] < 1 || m.n[1] < 1 || m.n[2] < 1) return fail(ctx, ASOHF_ERR_INVALID, "patch sizes must be >= 1");
+    if ((long long)m.n[0] * m.n[1] * m.n[2] > (1ll << 30)) return fail(ctx, ASOHF_ERR_CAPACITY, "patch too large");
+    if (padded && (m.n[0] > opts->u11_ld[0] || m.n[1] > opts->u11_ld[1] || m.n[2] > opts->u11_ld[2]))
+      return fail(ctx, ASOHF_ERR_INVALID, "a patch exceeds the padded U11 dimensions");
+    m.pc[0] = px ? px[p] : 0;
+    m.pc[1] = py ? py[p] : 0;
+    m.pc[2] = pz ? pz[p] : 0;
+    m.pare = pare ? pare[p] : 0;
+    m.self = (int)(p + 1);
+    m.pr[0] = g->patchrx[p];
+    m.pr[1] = g->patchry[p];
+    m.pr[2] = g->patchrz[p];
+    m.pad = 0;
+    m.off = padded ? p * slab : off;
+    off += (long long)m.n[0] * m.n[1] * m.n[2];
+  }
+  cells11 = padded ? P * slab : off;
+  return ASOHF_OK;
+}
+
+static asohf_status base_ld(asohf_ctx* ctx, const asohf_geom* g, const asohf_opts* opts, long long& b1,
+                            long long& b2) {
+  const int l1 = (opts && opts->u1_ld[0] > 0) ? opts->u1_ld[0] : g->nx;
+  const int l2 = (opts && opts->u1_ld[1] > 0) ? opts->u1_ld[1] : g->ny;
+  if (l1 < g->nx || l2 < g->ny) return fail(ctx, ASOHF_ERR_INVALID, "u1 leading dimensions below NX, NY");
+  b1 = l1;
+  b2 = (long long)l1 * l2;
+  return ASOHF_OK;
+}
+
+static asohf_status mask_error(asohf_ctx* ctx, hipStream_t s) {
+  int e = 0;
+  HIPCHK(hipMemcpyAsync(&e, ctx->mk_err.p, sizeof(int), hipMemcpyDeviceToHost, s), "reading the mask status");
+  HIPCHK(hipStreamSynchronize(s), "grid masks");
+  if (e & 1) return fail(ctx, ASOHF_ERR_INVALID, "PARE of a level >= 2 patch is not a patch");
+  if (e & 2) return fail(ctx, ASOHF_ERR_INVALID, "a mother cell (PATCHX..Z) falls outside its parent grid");
+  if (e & 4) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 512 coarser patches near one finer patch");
+  if (e & 8) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 16 patches of one level cover one cell");
+  return ASOHF_OK;
+}
+
+asohf_status asohf_hip_refinement_masks(asohf_ctx* ctx, const asohf_geom* geom, const int32_t* patchx,
+                                        const int32_t* patchy, const int32_t* patchz, const int32_t* pare,
+                                        int32_t* cr0amr, int32_t* cr0amr11, const asohf_opts* opts) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  if (!geom || !cr0amr) return fail(ctx, ASOHF_ERR_INVALID, "NULL argument");
+  std::vector<MaskPatch> mp;
+  std::vector<int> lo, ln;
+  long long s1p, s2p, cells11, b1, b2;
+  asohf_status st = mask_patches(ctx, geom, patchx, patchy, patchz, pare, opts, mp, lo, ln, s1p, s2p, cells11);
+  if (st != ASOHF_OK) return st;
+  if ((st = base_ld(ctx, geom, opts, b1, b2)) != ASOHF_OK) return st;
+  if (!mp.empty() && (!patchx || !patchy || !patchz || !pare || !cr0amr11))
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL PATCHX..Z, PARE or cr0amr11");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t s = opts ? (hipStream_t)opts->stream : (hipStream_t)0;
+  HIPCHK(ctx->mk_err.ensure(sizeof(int)), "allocating the mask status");
+  HIPCHK(launch_fill_i32(ctx->mk_err.as<int>(), 1, 0, ctx->cus, s), "clearing the mask status");
+  if (!mp.empty()) {
+    HIPCHK(ctx->mk_geo.ensure(sizeof(MaskPatch) * mp.size()), "allocating the patch records");
+    HIPCHK(hipMemcpyAsync(ctx->mk_geo.p, mp.data(), sizeof(MaskPatch) * mp.size(), hipMemcpyHostToDevice, s),
+           "uploading the patch records");
+  }
+  HIPCHK(launch_cr0amr(ctx->mk_geo.as<MaskPatch>(), (int)mp.size(), geom->nl >= 1 ? ln[1] : 0, lo.data(), ln.data(),
+                       geom->nl, geom->lado0, geom->nx, geom->ny, geom->nz, cr0amr, b1, b2, cr0amr11, cells11, s1p,
+                       s2p, ctx->mk_err.as<int>(), ctx->cus, s),
+         "refinement masks");
+  return mask_error(ctx, s);
+}
+
+asohf_status asohf_hip_overlap_mask(asohf_ctx* ctx, const asohf_geom* geom, int32_t* solap, const asohf_opts* opts) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  if (!geom) return fail(ctx, ASOHF_ERR_INVALID, "NULL argument");
+  std::vector<MaskPatch> mp;
+  std::vector<int> lo, ln;
+  long long s1p, s2p, cells11;
+  asohf_status st = mask_patches(ctx, geom, nullptr, nullptr, nullptr, nullptr, opts, mp, lo, ln, s1p, s2p, cells11);
+  if (st != ASOHF_OK) return st;
+  if (mp.empty()) return ASOHF_OK;
+  if (!solap) return fail(ctx, ASOHF_ERR_INVALID, "NULL solap");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t s = opts ? (hipStream_t)opts->stream : (hipStream_t)0;
+  HIPCHK(ctx->mk_err.ensure(sizeof(int)), "allocating the mask status");
+  HIPCHK(launch_fill_i32(ctx->mk_err.as<int>(), 1, 0, ctx->cus, s), "clearing the mask status");
+  HIPCHK(ctx->mk_geo.ensure(sizeof(MaskPatch) * mp.size()), "allocating the patch records");
+  HIPCHK(hipMemcpyAsync(ctx->mk_geo.p, mp.data(), sizeof(MaskPatch) * mp.size(), hipMemcpyHostToDevice, s),
+         "uploading the patch records");
+  // SOLAP(:,:,:,I) = 1 over the whole (padded) array of every patch
+  if (s1p > 0) HIPCHK(launch_fill_i32(solap, cells11, 1, ctx->cus, s), "filling SOLAP");
+  // RADX(1) and DX from COMMON /GRID/ and /ESPACIADO/ (MALLA's when RADX is not given)
+  const float lado = geom->lado0 - geom->lado0 / (float)geom->nx;
+  std::vector<float> rad[3];
+  const float* given[3] = {geom->radx, geom->rady, geom->radz};
+  const int nn[3] = {geom->nx, geom->ny, geom->nz};
+  for (int a = 0; a < 3; ++a) {
+    if (given[a]) rad[a].assign(given[a], given[a] + nn[a] + 2);
+    else malla(nn[a], lado, rad[a]);
+  }
+  const float dd[3] = {geom->dx, geom->dy, geom->dz};
+  std::vector<unsigned> cnt;
+  std::vector<unsigned long long> start;
+  for (int ir = 1; ir <= geom->nl; ++ir) {
+    const int np = ln[ir];
+    if (np == 0) continue;
+    VgGeo G;
+    float p2 = 1.0f;
+    for (int k = 0; k < ir; ++k) p2 *= 2.0f;
+    for (int a = 0; a < 3; ++a) {
+      G.d[a] = dd[a] / p2;                                     // DXPA = DX/(2.0**IR)
+      G.fix[a] = (rad[a][1] - dd[a] * 0.5f) + 0.5f * G.d[a];   // RXFIX (grids.f:2285-2287)
+    }
+    HIPCHK(ctx->vg_cnt.ensure(sizeof(unsigned) * np), "allocating the neighbour counts");
+    HIPCHK(ctx->vg_start.ensure(sizeof(unsigned long long) * (np + 1)), "allocating the neighbour offsets");
+    HIPCHK(launch_vg_neighbours(ctx->mk_geo.as<MaskPatch>(), lo[ir], np, G, 0, ctx->vg_cnt.as<unsigned>(), nullptr,
+                                nullptr, s),
+           "counting neighbours");
+    cnt.resize(np);
+    HIPCHK(hipMemcpyAsync(cnt.data(), ctx->vg_cnt.p, sizeof(unsigned) * np, hipMemcpyDeviceToHost, s),
+           "reading the neighbour counts");
+    HIPCHK(hipStreamSynchronize(s), "neighbour counts");
+    start.assign(np + 1, 0);
+    for (int k = 0; k < np; ++k) start[k + 1] = start[k] + cnt[k];
+    HIPCHK(hipMemcpyAsync(ctx->vg_start.p, start.data(), sizeof(unsigned long long) * (np + 1),
+                          hipMemcpyHostToDevice, s),
+           "uploading the neighbour offsets");
+    HIPCHK(ctx->vg_pairs.ensure(sizeof(VgPair) * std::max<unsigned long long>(start[np], 1)),
+           "allocating the neighbour pairs");
+    HIPCHK(launch_vg_neighbours(ctx->mk_geo.as<MaskPatch>(), lo[ir], np, G, 1, nullptr,
+                                ctx->vg_start.as<unsigned long long>(), ctx->vg_pairs.as<VgPair>(), s),
+           "listing neighbours");
+    HIPCHK(launch_vg_solap(ctx->mk_geo.as<MaskPatch>(), lo[ir], np, ctx->vg_start.as<unsigned long long>(),
+                           ctx->vg_pairs.as<VgPair>(), solap, s1p, s2p, ctx->mk_err.as<int>(), ctx->cus, s),
+           "overlap mask");
+    // the next level reuses the offsets and pairs
+    HIPCHK(hipStreamSynchronize(s), "overlap mask");
+  }
+  return mask_error(ctx, s);
+}
+
+// Unformatted sequential records as the reference's compiler writes them:
+// a 4-byte length before and after the data.  Records longer than
+// 2,147,483,639 bytes are split into subrecords whose leading marker is
+// negative when another subrecord follows and whose trailing marker is
+// negative when one precedes (the convention of gfortran's runtime).
+namespace {
+struct DumpWriter {
+  FILE* f = nullptr;
+  hipStream_t s = nullptr;
+  char* stage = nullptr;   // pinned staging buffer
+  size_t cap = 0;
+  bool ok = true;
+  static constexpr long long kSub = 2147483639ll;
+  long long rec_left = 0, sub_left = 0, sub_len = 0;
+  bool first_sub = true;
+  void marker(int32_t v) { ok = ok && fwrite(&v, 4, 1, f) == 1; }
+  void begin(long long bytes) {
+    rec_left = bytes;
+    first_sub = true;
+    open_sub();
+    if (bytes == 0) close_sub();
+  }
+  void open_sub() {
+    sub_len = std::min(rec_left, kSub);
+    sub_left = sub_len;
+    marker((int32_t)(rec_left > kSub ? -sub_len : sub_len));
+  }
+  void close_sub() {
+    marker((int32_t)(first_sub ? sub_len : -sub_len));
+    first_sub = false;
+  }
+  void put(const char* p, long long n) {
+    while (n > 0 && ok) {
+      const long long k = std::min(n, sub_left);
+      ok = ok && fwrite(p, 1, (size_t)k, f) == (size_t)k;
+      p += k;
+      n -= k;
+      sub_left -= k;
+      rec_left -= k;
+      if (sub_left == 0) {
+        close_sub();
+        if (rec_left > 0) open_sub();
+      }
+    }
+  }
+};
+}  // namespace
+
+asohf_status asohf_hip_write_density_dump(asohf_ctx* ctx, const char* path, const asohf_geom* geom, const float* u1,
+                                          const int32_t* cr0amr, const float* u11, const int32_t* cr0amr11,
+                                          const int32_t* solap, const asohf_opts* opts) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  if (!path || !geom || !u1 || !cr0amr) return fail(ctx, ASOHF_ERR_INVALID, "NULL argument");
+  std::vector<MaskPatch> mp;
+  std::vector<int> lo, ln;
+  long long s1p, s2p, cells11, b1, b2;
+  asohf_status st = mask_patches(ctx, geom, nullptr, nullptr, nullptr, nullptr, opts, mp, lo, ln, s1p, s2p, cells11);
+  if (st != ASOHF_OK) return st;
+  if ((st = base_ld(ctx, geom, opts, b1, b2)) != ASOHF_OK) return st;
+  if (!mp.empty() && (!u11 || !cr0amr11 || !solap)) return fail(ctx, ASOHF_ERR_INVALID, "NULL patch arrays");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  DumpWriter W;
+  W.s = opts ? (hipStream_t)opts->stream : (hipStream_t)0;
+  W.cap = (size_t)64 << 20;
+  HIPCHK(hipHostMalloc((void**)&W.stage, W.cap, hipHostMallocDefault), "allocating the dump staging buffer");
+  W.f = fopen(path, "wb");
+  if (!W.f) {
+    (void)hipHostFree(W.stage);
+    return fail(ctx, ASOHF_ERR_INVALID, std::string("cannot open ") + path);
+  }
+  std::vector<char> fbuf((size_t)8 << 20);
+  setvbuf(W.f, fbuf.data(), _IOFBF, fbuf.size());
+  hipError_t he = hipSuccess;
+  auto fetch = [&](const void* src, size_t bytes) {
+    he = hipMemcpyAsync(W.stage, src, bytes, hipMemcpyDeviceToHost, W.s);
+    if (he == hipSuccess) he = hipStreamSynchronize(W.s);
+    return he == hipSuccess;
+  };
+  const long long nx = geom->nx, ny = geom->ny, nz = geom->nz;
+  // U1 and CR0AMR: interiors, x fastest, one z-plane (or several) at a time
+  for (int which = 0; which < 2 && W.ok && he == hipSuccess; ++which) {
+    const char* base = which == 0 ? (const char*)u1 : (const char*)cr0amr;
+    W.begin(4 * nx * ny * nz);
+    const bool packed = b1 == nx && b2 == nx * ny;
+    if (packed) {
+      const long long total = 4 * nx * ny * nz;
+      for (long long o = 0; o < total && W.ok; o += (long long)W.cap) {
+        const long long k = std::min<long long>(W.cap, total - o);
+        if (!fetch(base + o, (size_t)k)) break;
+        W.put(W.stage, k);
+      }
+    } else {
+      const long long plane = 4 * b2;
+      if ((size_t)plane > W.cap) {
+        he = hipErrorInvalidValue;
+        break;
+      }
+      for (long long kz = 0; kz < nz && W.ok; ++kz) {
+        if (!fetch(base + kz * plane, (size_t)plane)) break;
+        for (long long jy = 0; jy < ny; ++jy) W.put(W.stage + 4 * b1 * jy, 4 * nx);
+      }
+    }
+  }
+  // per patch: U11, CR0AMR11, SOLAP, in batches of patches that fit a third
+  // of the staging buffer each
+  const size_t third = W.cap / 3;
+  size_t p = 0;
+  while (p < mp.size() && W.ok && he == hipSuccess) {
+    const long long first = mp[p].off;
+    size_t q = p;
+    long long span = 0;
+    while (q < mp.size()) {
+      const long long cells = s1p > 0 ? s2p * (opts ? opts->u11_ld[2] : 1)
+                                      : (long long)mp[q].n[0] * mp[q].n[1] * mp[q].n[2];
+      const long long end = mp[q].off + cells - first;
+      if ((size_t)(4 * end) > third) break;
+      span = end;
+      ++q;
+    }
+    if (q == p) {
+      he = hipErrorInvalidValue;   // one patch larger than the staging buffer
+      break;
+    }
+    const char* src[3] = {(const char*)(u11 + first), (const char*)(cr0amr11 + first), (const char*)(solap + first)};
+    for (int a = 0; a < 3 && he == hipSuccess; ++a) {
+      he = hipMemcpyAsync(W.stage + a * third, src[a], (size_t)(4 * span), hipMemcpyDeviceToHost, W.s);
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(W.s);
+    if (he != hipSuccess) break;
+    for (size_t r = p; r < q && W.ok; ++r) {
+      const MaskPatch& m = mp[r];
+      const long long rel = m.off - first;
+      for (int a = 0; a < 3; ++a) {
+        const char* b = W.stage + a * third + 4 * rel;
+        W.begin(4ll * m.n[0] * m.n[1] * m.n[2]);
+        if (s1p > 0) {
+          for (int kz = 0; kz < m.n[2]; ++kz)
+            for (int jy = 0; jy < m.n[1]; ++jy) W.put(b + 4 * (s1p * jy + s2p * kz), 4ll * m.n[0]);
+        } else {
+          W.put(b, 4ll * m.n[0] * m.n[1] * m.n[2]);
+        }
+      }
+    }
+    p = q;
+  }
+  const bool closed = fclose(W.f) == 0;
+  (void)hipHostFree(W.stage);
+  if (he != hipSuccess) return hip_fail(ctx, he, "copying the fields for the dump");
+  if (!W.ok || !closed) return fail(ctx, ASOHF_ERR_INVALID, std::string("write error on ") + path);
+  return ASOHF_OK;
+}
+
 asohf_status asohf_hip_release_workspaces(asohf_ctx* ctx) {
   if (!ctx) return ASOHF_ERR_INVALID;
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
@@ -1369,7 +1698,8 @@ asohf_status asohf_hip_release_workspaces(asohf_ctx* ctx) {
                   &ctx->ord[1], &ctx->ord[2], &ctx->ord[3], &ctx->ord_tmp, &ctx->elist, &ctx->eoff, &ctx->ecount,
                   &ctx->sp_k[0], &ctx->sp_k[1], &ctx->sp_k[2], &ctx->sp_k[3], &ctx->sp_dens, &ctx->sp_cnt,
                   &ctx->sp_tmp, &ctx->pk_r[0], &ctx->pk_r[1], &ctx->pk_k[0], &ctx->pk_k[1], &ctx->pk_tmp,
-                  &ctx->pk_cnt, &ctx->pk_geo};
+                  &ctx->pk_cnt, &ctx->pk_geo, &ctx->mk_geo, &ctx->mk_err, &ctx->vg_cnt, &ctx->vg_start,
+                  &ctx->vg_pairs};
   for (DBuf* b : bufs) b->release();
   ctx->sized = false;   // the next batched call sizes its buffers again
   return ASOHF_OK;

@@ -60,6 +60,32 @@ hipError_t launch_peak_sort_write(unsigned long long* rec_r, unsigned* rec_k, un
                                   unsigned* rec_k2, long long k, void* temp, size_t temp_bytes, int level, int nx,
                                   int ny, const PeakPatchRec* pp, int np, int patch0, float* dens, int* ix,
                                   int* iy, int* iz, int* patch, int cus, hipStream_t s);
+// Grid masks of the density dump (masks.hip)
+struct MaskPatch {
+  int n[3];        // PATCHNX, PATCHNY, PATCHNZ
+  int pc[3];       // PATCHX, PATCHY, PATCHZ
+  int pare;        // PARE (1-based, 0 = base grid)
+  int self;        // this patch's 1-based number
+  float pr[3];     // PATCHRX, PATCHRY, PATCHRZ
+  int pad;
+  long long off;   // first cell of the patch in the U11-layout arrays
+};
+struct VgGeo {
+  float d[3];      // DXPA, DYPA, DZPA of the level
+  float fix[3];    // RXFIX, RYFIX, RZFIX (grids.f:2285-2287)
+};
+struct VgPair {
+  int j;           // neighbour, index within the level
+  int c1[3], cc1[3], len[3];
+};
+hipError_t launch_cr0amr(const MaskPatch* mp, int np, int nlev1, const int* lev_lo, const int* lev_n, int nl,
+                         float lado0, int nx, int ny, int nz, int* cr0, long long b1, long long b2, int* cr11,
+                         long long cells11, long long s1p, long long s2p, int* err, int cus, hipStream_t s);
+hipError_t launch_vg_neighbours(const MaskPatch* mp, int lo, int np, const VgGeo& G, int pass, unsigned* count,
+                                const unsigned long long* start, VgPair* out, hipStream_t s);
+hipError_t launch_vg_solap(const MaskPatch* mp, int lo, int np, const unsigned long long* start, const VgPair* pairs,
+                           int* solap, long long s1p, long long s2p, int* err, int cus, hipStream_t s);
+hipError_t launch_fill_i32(int* p, long long n, int v, int cus, hipStream_t s);
 hipError_t launch_plain_zero(const Plan& P, float* u1, float* u11, hipStream_t s);
 hipError_t launch_plain_base(const Plan& P, const Parts& A, float* u1, int cus, hipStream_t s);
 hipError_t launch_plain_flags(const Plan& P, const Parts& A, int l, unsigned* flag, int cus, hipStream_t s);

@@ -14,6 +14,7 @@ library.  There is no CPU path.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import dataclasses
 
 import numpy as np
@@ -137,6 +138,45 @@ class DensityBuilder:
                 continue
             _lib.check(self._h, st)
             return tuple(t[: cnt.value] for t in outs)
+
+    def refinement_masks(self, geom: Geometry, *, u1_ld=None, u11_ld=None, device=None):
+        """COMPUTE_CR0AMR (grids.f:2076-2226): (cr0amr int32 [nz, ly, lx],
+        cr0amr11 int32 in u11's layout), 1 = not refined, 0 = refined."""
+        if geom.pcx is None or geom.pare is None:
+            raise ValueError("the geometry carries no PATCHX/Y/Z or PARE")
+        dev = torch.device(device) if device is not None else self.device
+        s1, n11 = self.out_shapes(geom, u1_ld, u11_ld)
+        c0 = torch.empty(s1, dtype=torch.int32, device=dev)
+        c11 = torch.empty(n11, dtype=torch.int32, device=dev)
+        pc = [np.ascontiguousarray(a, np.int32) for a in (geom.pcx, geom.pcy, geom.pcz, geom.pare)]
+        ga = _GeomArgs(geom)
+        o = self._opts("tsc", u1_ld=u1_ld, u11_ld=u11_ld)
+        st = _lib.load().asohf_hip_refinement_masks(self._h, ga.ref, *[_np_ptr(a, C.c_int32) for a in pc],
+                                                     _ptr(c0), _ptr(c11), C.byref(o))
+        _lib.check(self._h, st)
+        return c0, c11
+
+    def overlap_mask(self, geom: Geometry, *, u11_ld=None, device=None):
+        """VEINSGRID (grids.f:2229-2506) for every level: SOLAP int32 in
+        u11's layout."""
+        dev = torch.device(device) if device is not None else self.device
+        _, n11 = self.out_shapes(geom, None, u11_ld)
+        so = torch.empty(n11, dtype=torch.int32, device=dev)
+        ga = _GeomArgs(geom)
+        o = self._opts("tsc", u11_ld=u11_ld)
+        _lib.check(self._h, _lib.load().asohf_hip_overlap_mask(self._h, ga.ref, _ptr(so), C.byref(o)))
+        return so
+
+    def write_density_dump(self, path, geom: Geometry, u1, cr0amr, u11, cr0amr11, solap, *, u1_ld=None,
+                           u11_ld=None):
+        """The FW2 density dump of asohf.f:530-545 (unformatted sequential
+        records: U1, CR0AMR, then per patch U11, CR0AMR11, SOLAP)."""
+        ga = _GeomArgs(geom)
+        o = self._opts("tsc", u1_ld=u1_ld, u11_ld=u11_ld)
+        st = _lib.load().asohf_hip_write_density_dump(self._h, os.fsencode(str(path)), ga.ref, _ptr(u1),
+                                                       _ptr(cr0amr), _ptr(u11), _ptr(cr0amr11), _ptr(solap),
+                                                       C.byref(o))
+        _lib.check(self._h, st)
 
     def release_workspaces(self):
         """Free the device workspaces kept between calls (the plan stays)."""

@@ -69,6 +69,9 @@ class Geometry:
     pare: np.ndarray | None = None   # int32 [P], 1-based parent, 0 = base grid
     fdm: np.float32 = f32(1.0)
     interp_degree: int = 2
+    pcx: np.ndarray | None = None    # int32 [P], PATCHX/Y/Z: the parent cell (1-based)
+    pcy: np.ndarray | None = None    # holding the patch's first cells
+    pcz: np.ndarray | None = None
 
     def __post_init__(self):
         self.npatch = np.ascontiguousarray(self.npatch, dtype=np.int32)
@@ -78,6 +81,9 @@ class Geometry:
             setattr(self, k, np.ascontiguousarray(getattr(self, k), dtype=np.float32))
         if self.pare is not None:
             self.pare = np.ascontiguousarray(self.pare, dtype=np.int32)
+        for k in ("pcx", "pcy", "pcz"):
+            if getattr(self, k) is not None:
+                setattr(self, k, np.ascontiguousarray(getattr(self, k), dtype=np.int32))
         self.lado0 = f32(self.lado0)
         self.rote = f32(self.rote)
         self.rete = f32(self.rete)
@@ -125,7 +131,10 @@ class Geometry:
                         self.npatch[: levels + 1].copy(), self.pnx[:k], self.pny[:k], self.pnz[:k],
                         self.prx[:k], self.pry[:k], self.prz[:k],
                         pare=None if self.pare is None else self.pare[:k],
-                        fdm=self.fdm, interp_degree=self.interp_degree)
+                        fdm=self.fdm, interp_degree=self.interp_degree,
+                        pcx=None if self.pcx is None else self.pcx[:k],
+                        pcy=None if self.pcy is None else self.pcy[:k],
+                        pcz=None if self.pcz is None else self.pcz[:k])
 
     def denbas(self, ir: int) -> np.float32:
         """DENBAS of level ir in FP32: DXPA*DYPA*DZPA*ROTE*RETE**3 (grids.f:824, 844)."""

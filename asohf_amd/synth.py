@@ -287,19 +287,20 @@ def hierarchy(spec: SnapshotSpec, geom_base: dict, halos, rng: np.random.Generat
                             else:
                                 rr[a] = f32(levels[-1][pk]["r"][a] +
                                             (f32(i1[a]) - f32(1.5)) * dxp)               # grids.f:586-588
-                        cur.append({"n": nn, "r": rr, "parent": pk, "halo": h})
+                        cur.append({"n": nn, "r": rr, "parent": pk, "halo": h, "i1": i1})
         levels.append(cur)
     npatch = [0] + [len(l) for l in levels]
     base = np.cumsum([0] + npatch)
     flat = [p for l in levels for p in l]
     pn = np.array([p["n"] for p in flat], np.int64).reshape(-1, 3)
     pr = np.array([p["r"] for p in flat], np.float32).reshape(-1, 3)
+    pc = np.array([p["i1"] for p in flat], np.int64).reshape(-1, 3)   # PATCHX/Y/Z: parent cell
     pare = []
     for ir, l in enumerate(levels, start=1):
         for p in l:
             pare.append(0 if ir == 1 else p["parent"] + base[ir - 1] + 1)   # 1-based global parent
     return (np.array(npatch, np.int32), pn[:, 0], pn[:, 1], pn[:, 2], pr[:, 0], pr[:, 1], pr[:, 2],
-            np.array(pare, np.int64))
+            np.array(pare, np.int64), pc)
 
 
 def _tuned_hierarchy(spec, geom_base, halos, seed, n):
@@ -342,10 +343,11 @@ def make_snapshot(spec: SnapshotSpec, n_side: int | None = None, nx: int | None 
     lado0 = f32(spec.box)
     g0 = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, np.zeros(1, np.int32),
                   np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0))
-    npatch, pnx, pny, pnz, prx, pry, prz, pare = _tuned_hierarchy(
+    npatch, pnx, pny, pnz, prx, pry, prz, pare, pc = _tuned_hierarchy(
         spec, {"nx": spec.nx, "radx": g0.radx, "dx": g0.dx}, halos, spec.seed, n)
     geom = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, npatch, pnx, pny, pnz,
-                    prx, pry, prz, pare=pare, interp_degree=interp_degree)
+                    prx, pry, prz, pare=pare, interp_degree=interp_degree,
+                    pcx=pc[:, 0], pcy=pc[:, 1], pcz=pc[:, 2])
     x = pos[:, 0].astype(np.float32)
     y = pos[:, 1].astype(np.float32)
     z = pos[:, 2].astype(np.float32)
@@ -381,10 +383,11 @@ def make_snapshot_device(spec: SnapshotSpec, device="cuda:0", chunk: int = 1 << 
     lado0 = f32(spec.box)
     g0 = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, np.zeros(1, np.int32),
                   np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0), np.zeros(0))
-    npatch, pnx, pny, pnz, prx, pry, prz, pare = _tuned_hierarchy(
+    npatch, pnx, pny, pnz, prx, pry, prz, pare, pc = _tuned_hierarchy(
         spec, {"nx": spec.nx, "radx": g0.radx, "dx": g0.dx}, (nh, conc, centre, rvir), spec.seed, n)
     geom = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, npatch, pnx, pny, pnz,
-                    prx, pry, prz, pare=pare, interp_degree=interp_degree)
+                    prx, pry, prz, pare=pare, interp_degree=interp_degree,
+                    pcx=pc[:, 0], pcy=pc[:, 1], pcz=pc[:, 2])
 
     dev = torch.device(device)
     gen = torch.Generator(device=dev)

@@ -22,6 +22,8 @@ Prints ONE JSON line on stdout; progress goes to stderr.
   species    the species partition that feeds NPART_ESP (reader.f:140-336)
   peaks      the grid halo finder's peak candidates on every level of the
              field (haloes_grids.f:316-404, 700-790, SORT_CELLS_AMR order)
+  masks      the grid masks of the density dump (COMPUTE_CR0AMR, VEINSGRID)
+             and the FW2 dump writer (asohf.f:515-545)
   adjoint    the hand-written backward (gradients of sum(w * rho))
   plain_gpu  the same build as one launch per grid with global FP32 atomics
   cpu_baseline  the oracle's OpenMP restatement of the reference algorithm
@@ -71,6 +73,8 @@ def main() -> int:
     ap.add_argument("--peaks-steps", type=int, default=5,
                     help="calls of the peak-candidate scan over every level (0: skip)")
     ap.add_argument("--contrastec", type=float, default=200.0, help="CONTRASTEC of the peak-candidate scan")
+    ap.add_argument("--masks-steps", type=int, default=3,
+                    help="calls of the grid masks and the density dump writer (0: skip)")
     ap.add_argument("--plain-max-tests", type=float, default=5e12,
                     help="skip the plain form above this many particle-box tests")
     ap.add_argument("--tune", default="", help="tx,ty,tz,chunk,waves_per_cu[,block]: fixed tuning, no autotune")
@@ -296,6 +300,49 @@ def main() -> int:
                                             "kind": "port", "same_lists": bool(same)}
         log(f"[bench] peaks: {1000 * tp:.3f} ms/call over {g.nl + 1} levels, candidates {counts}")
         del res
+
+    if args.masks_steps > 0 and g.pcx is not None:
+        # the grid masks and the FW2 density dump (asohf.f:515-545)
+        import tempfile
+        r = bld.density(g, x, y, z, m, snap.npart_esp, **kw)
+        c0, c11 = bld.refinement_masks(g)
+        so = bld.overlap_mask(g)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.masks_steps):
+            c0, c11 = bld.refinement_masks(g)
+        torch.cuda.synchronize()
+        t_cr = (time.perf_counter() - t0) / args.masks_steps
+        t0 = time.perf_counter()
+        for _ in range(args.masks_steps):
+            so = bld.overlap_mask(g)
+        torch.cuda.synchronize()
+        t_vg = (time.perf_counter() - t0) / args.masks_steps
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "density_asohf00001.res")
+            t0 = time.perf_counter()
+            bld.write_density_dump(path, g, r.u1, c0, r.u11, c11, so)
+            t_w = time.perf_counter() - t0
+            size = os.path.getsize(path)
+        out["masks"] = {"ms_refinement_masks": 1000 * t_cr, "ms_overlap_mask": 1000 * t_vg,
+                        "dump_seconds": t_w, "dump_bytes": size, "dump_gbs": size / t_w / 1e9,
+                        "zeros": {"cr0amr": int((c0 == 0).sum()), "cr0amr11": int((c11 == 0).sum()),
+                                  "solap": int((so == 0).sum())},
+                        "routines": "COMPUTE_CR0AMR (grids.f:2076-2226), VEINSGRID (grids.f:2229-2506), "
+                                    "FW2 dump (asohf.f:530-545)"}
+        if not args.no_cpu and n_cells <= (1 << 26):
+            from oracle import oracle as O
+            t0 = time.perf_counter()
+            oc0, oc11 = O.cr0amr(g)
+            os_ = O.veinsgrid(g)
+            tc = time.perf_counter() - t0
+            out["masks"]["cpu_baseline"] = {
+                "seconds": tc, "cores": 1, "kind": "port",
+                "same_masks": bool(np.array_equal(oc0, c0.cpu().numpy()) and np.array_equal(oc11, c11.cpu().numpy())
+                                   and np.array_equal(os_, so.cpu().numpy()))}
+        log(f"[bench] masks: CR0AMR {1000 * t_cr:.3f} ms, SOLAP {1000 * t_vg:.3f} ms, "
+            f"dump {size / 1e6:.1f} MB in {t_w:.3f} s")
+        del c0, c11, so
 
     # the forward workspaces (sized by the largest mode above) are not needed by
     # the adjoint; free them so the adjoint's own buffers fit at any size

@@ -208,6 +208,41 @@ asohf_status asohf_hip_peak_candidates(asohf_ctx *ctx, const asohf_geom *geom, i
                                        int32_t *ix, int32_t *iy, int32_t *iz, int32_t *patch, int64_t cap,
                                        int64_t *count, const asohf_opts *opts);
 
+/* The grid masks of the density dump, as asohf.f:515-528 computes them
+ * before writing it.
+ * asohf_hip_refinement_masks: COMPUTE_CR0AMR (grids.f:2076-2226).  cr0amr
+ * (device int32, u1's layout: opts->u1_ld) and cr0amr11 (device int32,
+ * u11's layout: opts->u11_ld, every patch of levels 1..NL; the whole slab in
+ * the padded layout) receive 1 for cells that are not refined and 0 for
+ * the mother cells of finer patches and for cells whose centre lies within a
+ * finer patch that is not their child.  patchx/y/z (host int32 [P]):
+ * PATCHX..Z, the parent cell holding a patch's first cells; pare (host int32
+ * [P]): PARE, 1-based, 0 for level 1.  DX is LADO0/NX, as the routine
+ * computes it. */
+asohf_status asohf_hip_refinement_masks(asohf_ctx *ctx, const asohf_geom *geom, const int32_t *patchx,
+                                        const int32_t *patchy, const int32_t *patchz, const int32_t *pare,
+                                        int32_t *cr0amr, int32_t *cr0amr11, const asohf_opts *opts);
+
+/* asohf_hip_overlap_mask: VEINSGRID (grids.f:2229-2506) for IR = 1..NL.
+ * solap (device int32, u11's layout) receives 1, or 0 for a cell that
+ * another patch of its level also covers and keeps: overlaps resolved in
+ * the reference's sequential order (patches ascending, each with its
+ * neighbours ascending; a shared cell kept by both is dropped by the first
+ * when it lies on that patch's border, else by the second).  Uses RADX(1)
+ * (geom->radx, or MALLA's) and DX..DZ from geom. */
+asohf_status asohf_hip_overlap_mask(asohf_ctx *ctx, const asohf_geom *geom, int32_t *solap, const asohf_opts *opts);
+
+/* asohf_hip_write_density_dump: the FW2 density dump of asohf.f:530-545 at
+ * `path`, an unformatted sequential file (4-byte record markers): U1 and
+ * CR0AMR (interiors 1..NX x 1..NY x 1..NZ, x fastest), then for every patch
+ * U11, CR0AMR11 and SOLAP (interiors).  Inputs are device arrays in the
+ * layouts of opts (u1_ld for U1 and CR0AMR, u11_ld for the patch arrays).
+ * Records longer than 2,147,483,639 bytes are written as gfortran-style
+ * subrecords. */
+asohf_status asohf_hip_write_density_dump(asohf_ctx *ctx, const char *path, const asohf_geom *geom, const float *u1,
+                                          const int32_t *cr0amr, const float *u11, const int32_t *cr0amr11,
+                                          const int32_t *solap, const asohf_opts *opts);
+
 /* Frees the device workspaces the context keeps between calls (binned
  * entries, work items, block tables, emission lists, split tiles, the
  * adjoint's sort buffers, the captured graph); the geometry plan stays, and

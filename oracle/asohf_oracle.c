@@ -18,7 +18,10 @@
  *   - the peak candidates of HALOFIND_GRID, haloes_grids.f:316-404 (base)
  *     and 700-790 (AMR levels), sorted by SORT_CELLS / SORT_CELLS_AMR
  *     (haloes_grids.f:1341-1425) with ARGSORT (num.f:150-200)
- *     (orc_argsort32, orc_peaks32).
+ *     (orc_argsort32, orc_peaks32);
+ *   - the grid masks written with the density dump (asohf.f:515-545):
+ *     COMPUTE_CR0AMR grids.f:2076-2226 (orc_cr0amr) and VEINSGRID
+ *     grids.f:2229-2506 (orc_veinsgrid).
  *
  * Two arithmetic flavours:
  *   *32  -- the reference's own FP32 expressions, operand order and loop
@@ -1192,4 +1195,198 @@ int64_t orc_peaks32(int ng, const int32_t* n1, const int32_t* n2, const int32_t*
   free(d2);
   free(ind);
   return k;
+}
+
+/* COMPUTE_CR0AMR (grids.f:2076-2226) on packed patch arrays (patch p at
+ * off[p], x fastest).  cr0amr[nx*ny*nz] and cr0amr11 receive 1 for cells not
+ * refined, 0 for the mother cells of a finer patch: the level-1 patches'
+ * mothers on the base grid (PATCHX..Z = the first mother cell), level >= 2
+ * mothers in the parent patch PARE, and cells of level IR whose centre lies
+ * between the first and last cell centres of a level IR+1 patch that is not
+ * their child.  DX here is LADO0/NX, as the routine computes it.  Returns -1
+ * if a mother index falls outside its grid (the reference would write out
+ * of bounds). */
+int orc_cr0amr(int nx, int ny, int nz, int nl, const int32_t* npatch, const int32_t* pnx, const int32_t* pny,
+               const int32_t* pnz, const int32_t* px, const int32_t* py, const int32_t* pz, const int32_t* pare,
+               const float* prx, const float* pry, const float* prz, float lado0, int32_t* cr0amr,
+               int32_t* cr0amr11) {
+  int64_t ptot = 0;
+  for (int l = 1; l <= nl; ++l) ptot += npatch[l];
+  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ptot + 1));
+  off[0] = 0;
+  for (int64_t p = 0; p < ptot; ++p) off[p + 1] = off[p] + (int64_t)pnx[p] * pny[p] * pnz[p];
+  for (int64_t c = 0; c < (int64_t)nx * ny * nz; ++c) cr0amr[c] = 1;
+  for (int64_t c = 0; c < off[ptot]; ++c) cr0amr11[c] = 1;
+  int rc = 0;
+  /* level 1: mothers on the base grid (the loop steps by 2, grids.f:2123-2135) */
+  for (int64_t i = 0; i < (nl >= 1 ? npatch[1] : 0); ++i)
+    for (int kz = 1; kz <= pnz[i]; kz += 2)
+      for (int jy = 1; jy <= pny[i]; jy += 2)
+        for (int ix = 1; ix <= pnx[i]; ix += 2) {
+          const int c1 = px[i] - 1 + (ix + 1) / 2, c2 = py[i] - 1 + (jy + 1) / 2, c3 = pz[i] - 1 + (kz + 1) / 2;
+          if (c1 < 1 || c1 > nx || c2 < 1 || c2 > ny || c3 < 1 || c3 > nz) { rc = -1; continue; }
+          cr0amr[(size_t)(c1 - 1) + (size_t)nx * ((size_t)(c2 - 1) + (size_t)ny * (size_t)(c3 - 1))] = 0;
+        }
+  /* levels >= 2: every cell marks its mother in the parent (grids.f:2138-2160) */
+  int64_t low1 = nl >= 1 ? npatch[1] : 0;
+  for (int ir = 2; ir <= nl; ++ir) {
+    for (int64_t i = low1; i < low1 + npatch[ir]; ++i) {
+      const int64_t q = pare[i] - 1;
+      if (q < 0 || q >= ptot) { rc = -1; continue; }
+      for (int kz = 1; kz <= pnz[i]; ++kz)
+        for (int jy = 1; jy <= pny[i]; ++jy)
+          for (int ix = 1; ix <= pnx[i]; ++ix) {
+            const int c1 = px[i] - 1 + (ix + 1) / 2, c2 = py[i] - 1 + (jy + 1) / 2, c3 = pz[i] - 1 + (kz + 1) / 2;
+            if (c1 < 1 || c1 > pnx[q] || c2 < 1 || c2 > pny[q] || c3 < 1 || c3 > pnz[q]) { rc = -1; continue; }
+            cr0amr11[off[q] + (c1 - 1) + (int64_t)pnx[q] * ((c2 - 1) + (int64_t)pny[q] * (c3 - 1))] = 0;
+          }
+    }
+    low1 += npatch[ir];
+  }
+  /* cells inside a non-child finer patch (grids.f:2162-2224) */
+  const float dx = lado0 / (float)nx, dy = lado0 / (float)ny, dz = lado0 / (float)nz;
+  int64_t lo = 0;
+  for (int ir = 1; ir <= nl - 1; ++ir) {
+    float p2 = 1.0f;
+    for (int k = 0; k < ir; ++k) p2 *= 2.0f;
+    const float dxpa = dx / p2, dypa = dy / p2, dzpa = dz / p2;
+    const int64_t lo2 = lo + npatch[ir], hi2 = lo2 + npatch[ir + 1];
+    for (int64_t i = lo; i < lo + npatch[ir]; ++i)
+      for (int kz = 1; kz <= pnz[i]; ++kz)
+        for (int jy = 1; jy <= pny[i]; ++jy)
+          for (int ix = 1; ix <= pnx[i]; ++ix) {
+            int32_t* cell = &cr0amr11[off[i] + (ix - 1) + (int64_t)pnx[i] * ((jy - 1) + (int64_t)pny[i] * (kz - 1))];
+            if (*cell != 1) continue;
+            const float xx = (prx[i] - 0.5f * dxpa) + (float)(ix - 1) * dxpa;
+            const float yy = (pry[i] - 0.5f * dypa) + (float)(jy - 1) * dypa;
+            const float zz = (prz[i] - 0.5f * dzpa) + (float)(kz - 1) * dzpa;
+            for (int64_t ii = lo2; ii < hi2; ++ii) {
+              if (pare[ii] == (int32_t)(i + 1)) continue;
+              const float xx1 = prx[ii] - 0.5f * (dxpa * 0.5f), yy1 = pry[ii] - 0.5f * (dypa * 0.5f),
+                          zz1 = prz[ii] - 0.5f * (dzpa * 0.5f);
+              const float xx2 = xx1 + ((float)(pnx[ii] - 1) * dxpa) * 0.5f;
+              const float yy2 = yy1 + ((float)(pny[ii] - 1) * dypa) * 0.5f;
+              const float zz2 = zz1 + ((float)(pnz[ii] - 1) * dzpa) * 0.5f;
+              const float b1 = (xx - xx1) * (xx2 - xx), b2 = (yy - yy1) * (yy2 - yy), b3 = (zz - zz1) * (zz2 - zz);
+              if (b1 >= 0.0f && b2 >= 0.0f && b3 >= 0.0f) {
+                *cell = 0;
+                break;
+              }
+            }
+          }
+    lo += npatch[ir];
+  }
+  free(off);
+  return rc;
+}
+
+/* VEINSGRID (grids.f:2229-2506) for every level 1..nl, on packed patch
+ * arrays: SOLAP = 1, except where patches of one level overlap, resolved in
+ * the reference's sequential order (patch I ascending, its neighbours J
+ * ascending, cells KK, JJ, II): a cell kept by both I and J is dropped from
+ * I when it lies on I's border, else from J.  radx1..3 = RADX(1), RADY(1),
+ * RADZ(1) and dx..dz = COMMON /ESPACIADO/ (MALLA's).  Returns -1 if a mapped
+ * cell index leaves its patch. */
+static int vg_int(float v) { return (int)v; } /* Fortran INT: truncation */
+int orc_veinsgrid(int nl, const int32_t* npatch, const int32_t* pnx, const int32_t* pny, const int32_t* pnz,
+                  const float* prx, const float* pry, const float* prz, float dx, float dy, float dz, float radx1,
+                  float rady1, float radz1, int32_t* solap) {
+  int64_t ptot = 0;
+  for (int l = 1; l <= nl; ++l) ptot += npatch[l];
+  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ptot + 1));
+  off[0] = 0;
+  for (int64_t p = 0; p < ptot; ++p) off[p + 1] = off[p] + (int64_t)pnx[p] * pny[p] * pnz[p];
+  int rc = 0;
+  int64_t low1 = 0;
+  for (int ir = 1; ir <= nl; ++ir) {
+    const int64_t np = npatch[ir], lo = low1;
+    float p2 = 1.0f;
+    for (int k = 0; k < ir; ++k) p2 *= 2.0f;
+    const float dxpa = dx / p2, dypa = dy / p2, dzpa = dz / p2;
+    const float rxfix = (radx1 - dx * 0.5f) + 0.5f * dxpa;
+    const float ryfix = (rady1 - dy * 0.5f) + 0.5f * dypa;
+    const float rzfix = (radz1 - dz * 0.5f) + 0.5f * dzpa;
+    int* g = (int*)malloc(sizeof(int) * 6 * (size_t)(np > 0 ? np : 1));
+    for (int64_t k = 0; k < np; ++k) {
+      const int64_t i = lo + k;
+      const float rx1 = prx[i] - 0.5f * dxpa, ry1 = pry[i] - 0.5f * dypa, rz1 = prz[i] - 0.5f * dzpa;
+      g[6 * k] = vg_int(((rx1 - rxfix) / dxpa) + 0.5f) + 1;
+      g[6 * k + 1] = vg_int(((ry1 - ryfix) / dypa) + 0.5f) + 1;
+      g[6 * k + 2] = vg_int(((rz1 - rzfix) / dzpa) + 0.5f) + 1;
+      g[6 * k + 3] = g[6 * k] + pnx[i] - 1;
+      g[6 * k + 4] = g[6 * k + 1] + pny[i] - 1;
+      g[6 * k + 5] = g[6 * k + 2] + pnz[i] - 1;
+      for (int64_t c = off[i]; c < off[i + 1]; ++c) solap[c] = 1;
+    }
+    for (int64_t k = 0; k < np; ++k) {
+      const int64_t i = lo + k;
+      const int n1 = pnx[i], n2 = pny[i], n3 = pnz[i];
+      const float rx1 = prx[i] - 0.5f * dxpa, ry1 = pry[i] - 0.5f * dypa, rz1 = prz[i] - 0.5f * dzpa;
+      const float rx2 = (prx[i] - 0.5f * dxpa) + (float)(n1 - 1) * dxpa;
+      const float ry2 = (pry[i] - 0.5f * dypa) + (float)(n2 - 1) * dypa;
+      const float rz2 = (prz[i] - 0.5f * dzpa) + (float)(n3 - 1) * dzpa;
+      for (int64_t kk2 = 0; kk2 < np; ++kk2) {
+        if (kk2 == k) continue;
+        if (!(g[6 * k] <= g[6 * kk2 + 3] && g[6 * kk2] <= g[6 * k + 3] && g[6 * k + 1] <= g[6 * kk2 + 4] &&
+              g[6 * kk2 + 1] <= g[6 * k + 4] && g[6 * k + 2] <= g[6 * kk2 + 5] && g[6 * kk2 + 2] <= g[6 * k + 5]))
+          continue;
+        const int64_t j = lo + kk2;
+        const int nn1 = pnx[j], nn2 = pny[j], nn3 = pnz[j];
+        const float r1[3] = {rx1, ry1, rz1}, r2[3] = {rx2, ry2, rz2};
+        const float d[3] = {dxpa, dypa, dzpa};
+        const int nme[3] = {n1, n2, n3}, noth[3] = {nn1, nn2, nn3};
+        const float pr[3] = {prx[j], pry[j], prz[j]};
+        int c1[3], c2[3], cc1[3], cc2[3];
+        for (int a = 0; a < 3; ++a) {
+          const float q1 = pr[a] - 0.5f * d[a];
+          const float q2 = (pr[a] - 0.5f * d[a]) + (float)(noth[a] - 1) * d[a];
+          if (q1 >= r1[a] && q2 <= r2[a]) {
+            c1[a] = vg_int(((q1 - r1[a]) / d[a]) + 0.5f) + 1;
+            c2[a] = vg_int(((q2 - r1[a]) / d[a]) + 0.5f) + 1;
+            cc1[a] = 1;
+            cc2[a] = noth[a];
+          }
+          if (q1 >= r1[a] && q2 > r2[a]) {
+            c1[a] = vg_int(((q1 - r1[a]) / d[a]) + 0.5f) + 1;
+            c2[a] = nme[a];
+            cc1[a] = 1;
+            cc2[a] = vg_int(((r2[a] - q1) / d[a]) + 0.5f) + 1;
+          }
+          if (q2 <= r2[a] && q1 < r1[a]) {
+            c1[a] = 1;
+            c2[a] = vg_int(((q2 - r1[a]) / d[a]) + 0.5f) + 1;
+            cc1[a] = vg_int(((r1[a] - q1) / d[a]) + 0.5f) + 1;
+            cc2[a] = noth[a];
+          }
+          if (q1 < r1[a] && q2 > r2[a]) {
+            c1[a] = 1;
+            c2[a] = nme[a];
+            cc1[a] = vg_int(((r1[a] - q1) / d[a]) + 0.5f) + 1;
+            cc2[a] = vg_int(((r2[a] - q1) / d[a]) + 0.5f) + 1;
+          }
+        }
+        (void)c2;
+        for (int kz2 = cc1[2]; kz2 <= cc2[2]; ++kz2)
+          for (int jy2 = cc1[1]; jy2 <= cc2[1]; ++jy2)
+            for (int ix2 = cc1[0]; ix2 <= cc2[0]; ++ix2) {
+              const int ix = ix2 - cc1[0] + c1[0], jy = jy2 - cc1[1] + c1[1], kz = kz2 - cc1[2] + c1[2];
+              if (ix < 1 || ix > n1 || jy < 1 || jy > n2 || kz < 1 || kz > n3 || ix2 < 1 || ix2 > nn1 ||
+                  jy2 < 1 || jy2 > nn2 || kz2 < 1 || kz2 > nn3) {
+                rc = -1;
+                continue;
+              }
+              int32_t* si = &solap[off[i] + (ix - 1) + (int64_t)n1 * ((jy - 1) + (int64_t)n2 * (kz - 1))];
+              int32_t* sj = &solap[off[j] + (ix2 - 1) + (int64_t)nn1 * ((jy2 - 1) + (int64_t)nn2 * (kz2 - 1))];
+              if (*si == 1 && *sj == 1) {
+                if (ix == 1 || ix == n1 || jy == 1 || jy == n2 || kz == 1 || kz == n3) *si = 0;
+                else *sj = 0;
+              }
+            }
+      }
+    }
+    free(g);
+    low1 += np;
+  }
+  free(off);
+  return rc;
 }

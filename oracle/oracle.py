@@ -9,7 +9,8 @@ module, and only as the checker or the timed CPU baseline.  The product path
     compiled from the reference sources into _ref/asohf_ref_density
     (built by `make ref` where the reference tree exists); run_ref_species()
     and run_ref_sort_cells() call SORT_DM_PARTICLES_LOCALDENSITY and
-    SORT_CELLS_AMR through the same driver.
+    SORT_CELLS_AMR through the same driver, run_ref_dump() the grid masks
+    and the density dump of asohf.f:515-545.
 """
 from __future__ import annotations
 
@@ -372,3 +373,114 @@ def run_ref_sort_cells(ddd, dddx, dddy, dddz, dddp):
     d = np.frombuffer(raw[:4 * k].tobytes(), np.float32).copy()
     rest = [np.frombuffer(raw[4 * k * (i + 1):4 * k * (i + 2)].tobytes(), np.int32).copy() for i in range(4)]
     return (d, *rest)
+
+
+def _patch_arrays(geom):
+    need = (geom.pcx, geom.pcy, geom.pcz, geom.pare)
+    if any(a is None for a in need):
+        raise ValueError("the geometry carries no PATCHX/Y/Z or PARE")
+    return [np.ascontiguousarray(a, np.int32) for a in need]
+
+
+def cr0amr(geom):
+    """COMPUTE_CR0AMR (grids.f:2076-2226): (cr0amr int32 [nz, ny, nx],
+    cr0amr11 int32 packed per patch, x fastest)."""
+    px, py, pz, pare = _patch_arrays(geom)
+    c0 = np.empty(geom.ncells_base, np.int32)
+    c11 = np.empty(max(int(geom.patch_cells().sum()), 1), np.int32)
+    L = lib()
+    L.orc_cr0amr.restype = C.c_int
+    rc = L.orc_cr0amr(geom.nx, geom.ny, geom.nz, geom.nl, _p(geom.npatch, _ip), _p(geom.pnx, _ip),
+                      _p(geom.pny, _ip), _p(geom.pnz, _ip), _p(px, _ip), _p(py, _ip), _p(pz, _ip),
+                      _p(pare, _ip), _p(geom.prx, _fp), _p(geom.pry, _fp), _p(geom.prz, _fp),
+                      C.c_float(geom.lado0), _p(c0, _ip), _p(c11, _ip))
+    if rc != 0:
+        raise ValueError("a mother cell falls outside its grid")
+    return c0.reshape(geom.nz, geom.ny, geom.nx), c11[: int(geom.patch_cells().sum())]
+
+
+def veinsgrid(geom):
+    """VEINSGRID (grids.f:2229-2506) for levels 1..NL: SOLAP int32 packed."""
+    s = np.empty(max(int(geom.patch_cells().sum()), 1), np.int32)
+    L = lib()
+    L.orc_veinsgrid.restype = C.c_int
+    rc = L.orc_veinsgrid(geom.nl, _p(geom.npatch, _ip), _p(geom.pnx, _ip), _p(geom.pny, _ip),
+                         _p(geom.pnz, _ip), _p(geom.prx, _fp), _p(geom.pry, _fp), _p(geom.prz, _fp),
+                         C.c_float(geom.dx), C.c_float(geom.dy), C.c_float(geom.dz), C.c_float(geom.radx[1]),
+                         C.c_float(geom.rady[1]), C.c_float(geom.radz[1]), _p(s, _ip))
+    if rc != 0:
+        raise ValueError("an overlap maps a cell outside its patch")
+    return s[: int(geom.patch_cells().sum())]
+
+
+def dump_bytes(geom, u1, c0, u11, c11, solap) -> bytes:
+    """The FW2 density dump of asohf.f:530-545 as an unformatted sequential
+    file with 4-byte record markers: U1, CR0AMR, then per patch U11,
+    CR0AMR11, SOLAP (interiors, x fastest)."""
+    out = []
+
+    def rec(a):
+        b = np.ascontiguousarray(a).tobytes()
+        m = np.array([len(b)], np.int32).tobytes()
+        out.extend((m, b, m))
+    rec(np.asarray(u1, np.float32))
+    rec(np.asarray(c0, np.int32))
+    off = np.concatenate([[0], np.cumsum(geom.patch_cells())]).astype(np.int64)
+    for p in range(geom.npatch_total):
+        sl = slice(off[p], off[p + 1])
+        rec(np.asarray(u11[sl], np.float32))
+        rec(np.asarray(c11[sl], np.int32))
+        rec(np.asarray(solap[sl], np.int32))
+    return b"".join(out)
+
+
+def parse_dump(raw: bytes, geom):
+    """Records of a density dump: (u1, cr0amr, u11, cr0amr11, solap), the
+    patch arrays packed."""
+    recs, pos = [], 0
+    while pos < len(raw):
+        n = int(np.frombuffer(raw[pos:pos + 4], np.int32)[0])
+        recs.append(raw[pos + 4:pos + 4 + n])
+        assert raw[pos + 4 + n:pos + 8 + n] == raw[pos:pos + 4]
+        pos += 8 + n
+    u1 = np.frombuffer(recs[0], np.float32).reshape(geom.nz, geom.ny, geom.nx)
+    c0 = np.frombuffer(recs[1], np.int32).reshape(geom.nz, geom.ny, geom.nx)
+    P = geom.npatch_total
+    u11 = np.concatenate([np.frombuffer(recs[2 + 3 * p], np.float32) for p in range(P)]) if P else np.zeros(0, np.float32)
+    c11 = np.concatenate([np.frombuffer(recs[3 + 3 * p], np.int32) for p in range(P)]) if P else np.zeros(0, np.int32)
+    so = np.concatenate([np.frombuffer(recs[4 + 3 * p], np.int32) for p in range(P)]) if P else np.zeros(0, np.int32)
+    return u1, c0, u11, c11, so
+
+
+def run_ref_dump(snap) -> bytes:
+    """The reference's own sequence of asohf.f:515-545 on this snapshot
+    (INTERPOLATE_DENSITY_TSC, RENORM_DENSITY, VEINSGRID for every level,
+    COMPUTE_CR0AMR, the FW2 WRITE statements): the dump file's bytes."""
+    import resource
+    geom = snap.geom
+    x, y, z, m, esp = _arrs(snap)
+    px, py, pz, pare = _patch_arrays(geom)
+    P = geom.npatch_total
+
+    def big_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([5, geom.nx, geom.ny, geom.nz, geom.nl, len(x), geom.interp_degree, P], np.int32).tofile(f)
+            geom.npatch.astype(np.int32).tofile(f)
+            esp.astype(np.int32).tofile(f)
+            np.array([geom.lado0, geom.rote, geom.rete, geom.fdm], np.float32).tofile(f)
+            for a in (geom.pnx, geom.pny, geom.pnz):
+                a.astype(np.int32).tofile(f)
+            for a in (geom.prx, geom.pry, geom.prz):
+                a.astype(np.float32).tofile(f)
+            for a in (x, y, z, m):
+                a.tofile(f)
+            for a in (px, py, pz, pare):
+                a.tofile(f)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([REF_BIN, fin, fout], check=True, stdout=subprocess.DEVNULL, env=env,
+                       preexec_fn=big_stack)
+        with open(fout, "rb") as f:
+            return f.read()

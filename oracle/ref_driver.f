@@ -14,6 +14,14 @@
 *               K = hdr(6) candidate cells: input real*4 ddd(K), int32
 *               dddx(K), dddy(K), dddz(K), dddp(K) after the header;
 *               output the same five arrays after the sort
+*       mode 5: as mode 2, then the grid masks and the density dump of
+*               asohf.f:515-545: VEINSGRID for IR = 1..NL (grids.f:
+*               2229-2506), COMPUTE_CR0AMR (grids.f:2076-2226) and the
+*               FW2 records (U1, CR0AMR, then per patch U11, CR0AMR11,
+*               SOLAP) written to the output file as an unformatted
+*               sequential file.  Extra input after the particles: int32
+*               patchx(P), patchy(P), patchz(P), pare(P).  Needs a large
+*               stack (VEINSGRID's VECINO is NPALEV x NPALEV).
 *     The grid constants come from the reference's MALLA (grids.f:2509)
 *     called with LADO = LADO0 - LADO0/NX as asohf.f:323 does.
 *
@@ -54,6 +62,10 @@
       REAL*4 SCAL(4),LADO0,LADO,FDM,T,ZETA
       CHARACTER*512 FIN,FOUT
       REAL*4, ALLOCATABLE :: DDD(:)
+      INTEGER SOLAP(NAMRX,NAMRY,NAMRZ,NPALEV)
+      INTEGER CR0AMR(NMAX,NMAY,NMAZ)
+      INTEGER CR0AMR11(NAMRX,NAMRY,NAMRZ,NPALEV)
+      INTEGER IR
       INTEGER, ALLOCATABLE :: DDDX(:),DDDY(:),DDDZ(:),DDDP(:)
 
       CALL GET_COMMAND_ARGUMENT(1,FIN)
@@ -129,6 +141,12 @@
        READ(10) (RYPA(I),I=1,N)
        READ(10) (RZPA(I),I=1,N)
        READ(10) (MASAP(I),I=1,N)
+       IF (MODE.EQ.5) THEN
+        READ(10) (PATCHX(I),I=1,NPT)
+        READ(10) (PATCHY(I),I=1,NPT)
+        READ(10) (PATCHZ(I),I=1,NPT)
+        READ(10) (PARE(I),I=1,NPT)
+       END IF
       END IF
       CLOSE(10)
 
@@ -166,6 +184,30 @@
       END IF
       CALL RENORM_DENSITY(NL,NX,NY,NZ,NPATCH,PATCHNX,PATCHNY,
      &                    PATCHNZ,U1,U11,LADO0,0.0,0.0,FDM)
+
+      IF (MODE.EQ.5) THEN
+       DO IR=1,NL
+        CALL VEINSGRID(IR,NL,NPATCH,PARE,PATCHNX,PATCHNY,PATCHNZ,
+     &                 PATCHX,PATCHY,PATCHZ,PATCHRX,PATCHRY,PATCHRZ,
+     &                 SOLAP)
+       END DO
+       CALL COMPUTE_CR0AMR(NL,NX,NY,NZ,NPATCH,PARE,PATCHNX,PATCHNY,
+     &                     PATCHNZ,PATCHX,PATCHY,PATCHZ,PATCHRX,
+     &                     PATCHRY,PATCHRZ,CR0AMR,CR0AMR11,LADO0)
+       OPEN(99,FILE=FOUT,STATUS='UNKNOWN',FORM='UNFORMATTED')
+       WRITE(99) (((U1(IX,JY,KZ),IX=1,NX),JY=1,NY),KZ=1,NZ)
+       WRITE(99) (((CR0AMR(IX,JY,KZ),IX=1,NX),JY=1,NY),KZ=1,NZ)
+       DO IP=1,NPT
+        N1=PATCHNX(IP)
+        N2=PATCHNY(IP)
+        N3=PATCHNZ(IP)
+        WRITE(99) (((U11(IX,JY,KZ,IP),IX=1,N1),JY=1,N2),KZ=1,N3)
+        WRITE(99) (((CR0AMR11(IX,JY,KZ,IP),IX=1,N1),JY=1,N2),KZ=1,N3)
+        WRITE(99) (((SOLAP(IX,JY,KZ,IP),IX=1,N1),JY=1,N2),KZ=1,N3)
+       END DO
+       CLOSE(99)
+       STOP
+      END IF
 
       OPEN(11,FILE=FOUT,ACCESS='STREAM',FORM='UNFORMATTED',
      &     STATUS='REPLACE',ACTION='WRITE')
