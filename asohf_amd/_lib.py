@@ -26,7 +26,7 @@ EXPORTS = ("asohf_hip_create", "asohf_hip_destroy", "asohf_hip_last_error", "aso
            "asohf_hip_get_tune", "asohf_hip_set_tune", "asohf_hip_autotune", "asohf_hip_get_stats",
            "asohf_hip_release_workspaces", "asohf_hip_species", "asohf_hip_peak_candidates",
            "asohf_hip_refinement_masks", "asohf_hip_overlap_mask", "asohf_hip_write_density_dump",
-           "asohf_hip_version")
+           "asohf_hip_create_mesh", "asohf_hip_version")
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
@@ -65,6 +65,13 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class MeshParams(C.Structure):
+    _fields_ = [("nl_mesh", C.c_int32), ("refine_thr", C.c_int32), ("min_patchsize", C.c_int32),
+                ("frac_refinable", C.c_float), ("bor", C.c_int32), ("boramr", C.c_int32),
+                ("bor_ovlp", C.c_int32), ("namr", C.c_int32 * 3), ("npalev", C.c_int32),
+                ("mpapolev", C.c_int32 * 9)]
 
 
 class AsohfError(RuntimeError):
@@ -111,6 +118,8 @@ def load():
         L.asohf_hip_refinement_masks.argtypes = [vp, G, _ip, _ip, _ip, _ip, vp, vp, O]
         L.asohf_hip_overlap_mask.argtypes = [vp, G, vp, O]
         L.asohf_hip_write_density_dump.argtypes = [vp, C.c_char_p, G, vp, vp, vp, vp, vp, O]
+        L.asohf_hip_create_mesh.argtypes = [vp, G, C.c_int64, vp, vp, vp, _ip, C.POINTER(MeshParams), _ip, _ip,
+                                            _ip, _ip, _ip, _ip, _ip, _ip, _ip, _fp, _fp, _fp, C.c_int64, O]
         for nm in EXPORTS:
             if nm not in ("asohf_hip_destroy", "asohf_hip_last_error", "asohf_hip_version"):
                 getattr(L, nm).restype = C.c_int32

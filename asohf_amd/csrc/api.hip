@@ -1684,6 +1684,184 @@ asohf_status asohf_hip_write_density_dump(asohf_ctx* ctx, const char* path, cons
   return ASOHF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The mesh builder (CREATE_MESH)
+// ---------------------------------------------------------------------------
+asohf_status asohf_hip_create_mesh(asohf_ctx* ctx, const asohf_geom* base, int64_t n, const float* rx,
+                                   const float* ry, const float* rz, const int32_t* npart_esp,
+                                   const asohf_mesh_params* prm, int32_t* nl_out, int32_t* npatch_out,
+                                   int32_t* patchnx, int32_t* patchny, int32_t* patchnz, int32_t* patchx,
+                                   int32_t* patchy, int32_t* patchz, int32_t* pare, float* patchrx, float* patchry,
+                                   float* patchrz, int64_t cap, const asohf_opts* opts) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  if (!base || !prm || !npart_esp || !nl_out || !npatch_out || n < 0 || (n > 0 && (!rx || !ry || !rz)))
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL argument");
+  if (cap > 0 && (!patchnx || !patchny || !patchnz || !patchx || !patchy || !patchz || !pare || !patchrx ||
+                  !patchry || !patchrz))
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL patch output");
+  if (base->nx < 3 || base->ny < 3 || base->nz < 3) return fail(ctx, ASOHF_ERR_INVALID, "grid below 3 cells");
+  if (prm->nl_mesh < 1 || prm->nl_mesh > kNLevels) return fail(ctx, ASOHF_ERR_INVALID, "NL_MESH outside 1..9");
+  for (int a = 0; a < 3; ++a)
+    if (prm->namr[a] < 4) return fail(ctx, ASOHF_ERR_INVALID, "NAMRX..Z below 4");
+  if (prm->npalev < 1) return fail(ctx, ASOHF_ERR_INVALID, "NPALEV below 1");
+  if (n >= (1ll << 31)) return fail(ctx, ASOHF_ERR_CAPACITY, "2^31 or more particles");
+  MeshParams pr;
+  pr.nl_mesh = prm->nl_mesh;
+  pr.refine_thr = prm->refine_thr;
+  pr.min_patchsize = prm->min_patchsize;
+  pr.frac_refinable = prm->frac_refinable;
+  pr.bor = prm->bor;
+  pr.boramr = prm->boramr;
+  pr.bor_ovlp = prm->bor_ovlp;
+  for (int a = 0; a < 3; ++a) pr.namr[a] = prm->namr[a];
+  pr.npalev = prm->npalev;
+  for (int l = 0; l < kNLevels; ++l) pr.mpapolev[l] = prm->mpapolev[l];
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t s = opts ? (hipStream_t)opts->stream : (hipStream_t)0;
+  const int nx = base->nx, ny = base->ny, nz = base->nz;
+  const float lado = base->lado0 - base->lado0 / (float)nx;
+  std::vector<float> rad[3];
+  const float* given[3] = {base->radx, base->rady, base->radz};
+  const int nn[3] = {nx, ny, nz};
+  for (int a = 0; a < 3; ++a) {
+    if (given[a]) rad[a].assign(given[a], given[a] + nn[a] + 2);
+    else malla(nn[a], lado, rad[a]);
+  }
+  Parts A;
+  memset(&A, 0, sizeof(A));
+  A.x = rx;
+  A.y = ry;
+  A.z = rz;
+  A.m = rx;   // never read
+  A.n = n;
+  long long acc = 0;
+  for (int k = 0; k < kNEsp; ++k) {
+    if (npart_esp[k] < 0) return fail(ctx, ASOHF_ERR_INVALID, "negative NPART_ESP entry");
+    acc += npart_esp[k];
+    A.sp[k + 1] = acc;
+  }
+  // ---- level 1: base-grid counts on the GPU, growth on the host ----
+  const long long nc = (long long)nx * ny * nz;
+  HIPCHK(ctx->mk_geo.ensure(sizeof(int) * (size_t)nc), "allocating the base counts");
+  int* dcount = ctx->mk_geo.as<int>();
+  HIPCHK(launch_fill_i32(dcount, nc, 0, ctx->cus, s), "clearing the base counts");
+  if (n > 0)
+    HIPCHK(launch_mesh_count_base(A, nx, ny, nz, base->dx, base->dy, base->dz, pr.refine_thr, dcount, ctx->cus, s),
+           "base-grid counts");
+  std::vector<int> conta((size_t)nc);
+  HIPCHK(hipMemcpyAsync(conta.data(), dcount, sizeof(int) * (size_t)nc, hipMemcpyDeviceToHost, s),
+         "reading the base counts");
+  HIPCHK(hipStreamSynchronize(s), "base-grid counts");
+  std::vector<MeshPatchHost> all;
+  std::string e = mesh_first_level(nx, ny, nz, rad[0].data(), rad[1].data(), rad[2].data(), conta, pr, all);
+  if (!e.empty()) return fail(ctx, ASOHF_ERR_INVALID, e);
+  std::vector<int> npatch(kNLevels + 1, 0);
+  npatch[1] = (int)all.size();
+  const float rad1[3] = {rad[0][1], rad[1][1], rad[2][1]};
+  // ---- levels 2..NL_MESH: counts in every patch of the level above ----
+  for (int irpa = 1; irpa <= pr.nl_mesh - 1; ++irpa) {
+    if (npatch[irpa] == 0) break;
+    const int total = (int)all.size();
+    const int first = total - npatch[irpa];
+    // the geometry so far, for the level hash of the density plan
+    std::vector<int32_t> np_(irpa + 1, 0), gx(total), gy(total), gz(total);
+    std::vector<float> fx(total), fy(total), fz(total);
+    for (int l = 1; l <= irpa; ++l) np_[l] = npatch[l];
+    for (int p = 0; p < total; ++p) {
+      gx[p] = all[p].n[0];
+      gy[p] = all[p].n[1];
+      gz[p] = all[p].n[2];
+      fx[p] = all[p].pr[0];
+      fy[p] = all[p].pr[1];
+      fz[p] = all[p].pr[2];
+    }
+    asohf_geom g = *base;
+    g.nl = irpa;
+    g.npatch = np_.data();
+    g.patchnx = gx.data();
+    g.patchny = gy.data();
+    g.patchnz = gz.data();
+    g.patchrx = fx.data();
+    g.patchry = fy.data();
+    g.patchrz = fz.data();
+    asohf_opts o = default_opts();
+    o.mode = ASOHF_MODE_TSC;
+    asohf_status st = check_geom(ctx, &g, ASOHF_MODE_TSC);
+    if (st != ASOHF_OK) return st;
+    st = build_plan(ctx, &g, ASOHF_MODE_TSC, &o);
+    if (st != ASOHF_OK) return st;
+    std::vector<MeshPatchDev> md((size_t)total);
+    std::vector<long long> off((size_t)npatch[irpa] + 1, 0);
+    for (int p = 0; p < total; ++p) {
+      for (int a = 0; a < 3; ++a) {
+        md[p].n[a] = p >= first ? all[p].n[a] : 0;   // only this level's patches are counted into
+        md[p].pr[a] = all[p].pr[a];
+      }
+      md[p].off = 0;
+      if (p >= first) {
+        md[p].off = off[p - first];
+        off[p - first + 1] = off[p - first] + (long long)all[p].n[0] * all[p].n[1] * all[p].n[2];
+      }
+    }
+    const long long cells = off[npatch[irpa]];
+    HIPCHK(ctx->vg_pairs.ensure(sizeof(MeshPatchDev) * md.size()), "allocating the patch records");
+    HIPCHK(hipMemcpyAsync(ctx->vg_pairs.p, md.data(), sizeof(MeshPatchDev) * md.size(), hipMemcpyHostToDevice, s),
+           "uploading the patch records");
+    HIPCHK(ctx->mk_geo.ensure(sizeof(int) * (size_t)std::max(cells, 1ll)), "allocating the patch counts");
+    dcount = ctx->mk_geo.as<int>();
+    HIPCHK(launch_fill_i32(dcount, cells, 0, ctx->cus, s), "clearing the patch counts");
+    float p2 = 1.0f;
+    for (int k = 0; k < irpa; ++k) p2 *= 2.0f;
+    if (n > 0)
+      HIPCHK(launch_mesh_count_level(ctx->plan.dev, A, irpa, ctx->vg_pairs.as<MeshPatchDev>(), base->dx / p2,
+                                     base->dy / p2, base->dz / p2, pr.refine_thr, dcount, ctx->cus, s),
+             "patch counts");
+    std::vector<int> c11((size_t)cells);
+    HIPCHK(hipMemcpyAsync(c11.data(), dcount, sizeof(int) * (size_t)cells, hipMemcpyDeviceToHost, s),
+           "reading the patch counts");
+    HIPCHK(hipStreamSynchronize(s), "patch counts");
+    // BORAMR border (grids.f:391-401)
+    for (int k = 0; k < npatch[irpa]; ++k) {
+      const MeshPatchHost& P = all[(size_t)(first + k)];
+      for (int kz = 1; kz <= P.n[2]; ++kz)
+        for (int jy = 1; jy <= P.n[1]; ++jy)
+          for (int ix = 1; ix <= P.n[0]; ++ix)
+            if (ix <= pr.boramr || ix >= P.n[0] - pr.boramr + 1 || jy <= pr.boramr || jy >= P.n[1] - pr.boramr + 1 ||
+                kz <= pr.boramr || kz >= P.n[2] - pr.boramr + 1)
+              c11[(size_t)off[k] + (ix - 1) + (size_t)P.n[0] * ((jy - 1) + (size_t)P.n[1] * (kz - 1))] = 0;
+    }
+    std::vector<MeshPatchHost> next;
+    e = mesh_next_level(irpa, all, first, npatch[irpa], c11, off, base->dx, base->dy, base->dz, rad1, pr, total,
+                        next);
+    if (!e.empty()) return fail(ctx, ASOHF_ERR_CAPACITY, e);
+    npatch[irpa + 1] = (int)next.size();
+    all.insert(all.end(), next.begin(), next.end());
+  }
+  int nl = pr.nl_mesh;
+  for (int l = 1; l <= pr.nl_mesh; ++l)
+    if (npatch[l] == 0) {
+      nl = l - 1;
+      break;
+    }
+  *nl_out = nl;
+  for (int l = 0; l <= kNLevels; ++l) npatch_out[l] = npatch[l];
+  if ((int64_t)all.size() > cap) return fail(ctx, ASOHF_ERR_CAPACITY, "more patches than cap (npatch set)");
+  for (size_t p = 0; p < all.size(); ++p) {
+    patchnx[p] = all[p].n[0];
+    patchny[p] = all[p].n[1];
+    patchnz[p] = all[p].n[2];
+    patchx[p] = all[p].pc[0];
+    patchy[p] = all[p].pc[1];
+    patchz[p] = all[p].pc[2];
+    pare[p] = all[p].pare;
+    patchrx[p] = all[p].pr[0];
+    patchry[p] = all[p].pr[1];
+    patchrz[p] = all[p].pr[2];
+  }
+  return ASOHF_OK;
+}
+
 asohf_status asohf_hip_release_workspaces(asohf_ctx* ctx) {
   if (!ctx) return ASOHF_ERR_INVALID;
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");

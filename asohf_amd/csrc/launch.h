@@ -2,6 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+#include <vector>
+
 #include "internal.h"
 
 namespace asohf {
@@ -86,6 +89,36 @@ hipError_t launch_vg_neighbours(const MaskPatch* mp, int lo, int np, const VgGeo
 hipError_t launch_vg_solap(const MaskPatch* mp, int lo, int np, const unsigned long long* start, const VgPair* pairs,
                            int* solap, long long s1p, long long s2p, int* err, int cus, hipStream_t s);
 hipError_t launch_fill_i32(int* p, long long n, int v, int cus, hipStream_t s);
+// Mesh builder (mesh.hip)
+struct MeshParams {
+  int nl_mesh, refine_thr, min_patchsize;
+  float frac_refinable;
+  int bor, boramr, bor_ovlp;
+  int namr[3];
+  int npalev;
+  int mpapolev[kNLevels];
+};
+struct MeshPatchHost {
+  int n[3];        // PATCHNX..Z
+  int pc[3];       // PATCHX..Z
+  int pare;        // PARE
+  float pr[3];     // PATCHRX..Z
+};
+struct MeshPatchDev {
+  int n[3];
+  float pr[3];
+  long long off;   // first count of the patch (level being counted)
+};
+hipError_t launch_mesh_count_base(const Parts& A, int nx, int ny, int nz, float dx, float dy, float dz, int thr,
+                                  int* conta, int cus, hipStream_t s);
+hipError_t launch_mesh_count_level(const Plan& P, const Parts& A, int l, const MeshPatchDev* mp, float dxpa,
+                                   float dypa, float dzpa, int thr, int* conta, int cus, hipStream_t s);
+std::string mesh_first_level(int nx, int ny, int nz, const float* radx, const float* rady, const float* radz,
+                             std::vector<int>& conta1, const MeshParams& pr, std::vector<MeshPatchHost>& out);
+std::string mesh_next_level(int irpa, const std::vector<MeshPatchHost>& all, int first, int np,
+                            std::vector<int>& conta, const std::vector<long long>& off, float dx, float dy, float dz,
+                            const float rad1[3], const MeshParams& pr, int total_before,
+                            std::vector<MeshPatchHost>& out);
 hipError_t launch_plain_zero(const Plan& P, float* u1, float* u11, hipStream_t s);
 hipError_t launch_plain_base(const Plan& P, const Parts& A, float* u1, int cus, hipStream_t s);
 hipError_t launch_plain_flags(const Plan& P, const Parts& A, int l, unsigned* flag, int cus, hipStream_t s);

@@ -178,6 +178,43 @@ class DensityBuilder:
                                                        C.byref(o))
         _lib.check(self._h, st)
 
+    def create_mesh(self, base: Geometry, x, y, z, npart_esp, *, nl_mesh=4, refine_thr=3, min_patchsize=14,
+                    frac_refinable=0.05, bor=4, boramr=0, bor_ovlp=0, namr=(32, 32, 32), npalev=10000,
+                    mpapolev=None, cap=None):
+        """CREATE_MESH (grids.f:2-707): the AMR hierarchy the reference builds
+        for these particles (device FP32, reference order; npart_esp gives
+        the species blocks).  Defaults are asohf.dat's mesh block and the
+        reference build's NAMRX..Z = 32, NPALEV = 10000.  Returns a Geometry
+        on `base`'s grid with PATCHX..Z and PARE."""
+        n = _check_particles(x, y, z)
+        prm = _lib.MeshParams()
+        prm.nl_mesh, prm.refine_thr, prm.min_patchsize = nl_mesh, refine_thr, min_patchsize
+        prm.frac_refinable, prm.bor, prm.boramr, prm.bor_ovlp = frac_refinable, bor, boramr, bor_ovlp
+        prm.namr[0], prm.namr[1], prm.namr[2] = namr
+        prm.npalev = npalev
+        mp = [npalev] * 9 if mpapolev is None else list(mpapolev)
+        for k in range(9):
+            prm.mpapolev[k] = int(mp[k])
+        esp = np.ascontiguousarray(npart_esp, np.int32)
+        cap = npalev if cap is None else cap
+        ints = [np.zeros(max(cap, 1), np.int32) for _ in range(7)]
+        flts = [np.zeros(max(cap, 1), np.float32) for _ in range(3)]
+        nl = C.c_int32()
+        npatch = np.zeros(10, np.int32)
+        ga = _GeomArgs(base)
+        o = self._opts("tsc")
+        st = _lib.load().asohf_hip_create_mesh(
+            self._h, ga.ref, n, _ptr(x), _ptr(y), _ptr(z), _np_ptr(esp, C.c_int32), C.byref(prm), C.byref(nl),
+            _np_ptr(npatch, C.c_int32), *[_np_ptr(a, C.c_int32) for a in ints],
+            *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in flts], cap, C.byref(o))
+        _lib.check(self._h, st)
+        P = int(npatch[1:nl.value + 1].sum())
+        pnx, pny, pnz, px, py, pz, pare = (a[:P] for a in ints)
+        prx, pry, prz = (a[:P] for a in flts)
+        return Geometry(base.nx, base.ny, base.nz, base.lado0, base.rote, base.rete,
+                        npatch[:nl.value + 1].copy(), pnx, pny, pnz, prx, pry, prz, pare=pare,
+                        fdm=base.fdm, interp_degree=base.interp_degree, pcx=px, pcy=py, pcz=pz)
+
     def release_workspaces(self):
         """Free the device workspaces kept between calls (the plan stays)."""
         _lib.check(self._h, _lib.load().asohf_hip_release_workspaces(self._h))

@@ -22,6 +22,7 @@ Prints ONE JSON line on stdout; progress goes to stderr.
   species    the species partition that feeds NPART_ESP (reader.f:140-336)
   peaks      the grid halo finder's peak candidates on every level of the
              field (haloes_grids.f:316-404, 700-790, SORT_CELLS_AMR order)
+  create_mesh  the mesh builder CREATE_MESH on the same particles
   masks      the grid masks of the density dump (COMPUTE_CR0AMR, VEINSGRID)
              and the FW2 dump writer (asohf.f:515-545)
   adjoint    the hand-written backward (gradients of sum(w * rho))
@@ -73,6 +74,7 @@ def main() -> int:
     ap.add_argument("--peaks-steps", type=int, default=5,
                     help="calls of the peak-candidate scan over every level (0: skip)")
     ap.add_argument("--contrastec", type=float, default=200.0, help="CONTRASTEC of the peak-candidate scan")
+    ap.add_argument("--mesh-steps", type=int, default=2, help="calls of the mesh builder (0: skip)")
     ap.add_argument("--masks-steps", type=int, default=3,
                     help="calls of the grid masks and the density dump writer (0: skip)")
     ap.add_argument("--plain-max-tests", type=float, default=5e12,
@@ -300,6 +302,23 @@ def main() -> int:
                                             "kind": "port", "same_lists": bool(same)}
         log(f"[bench] peaks: {1000 * tp:.3f} ms/call over {g.nl + 1} levels, candidates {counts}")
         del res
+
+    if args.mesh_steps > 0:
+        # the mesh builder CREATE_MESH (grids.f:2-707) on the same particles,
+        # asohf.dat's mesh block, as many levels as the workload has
+        base = g.truncated(0)
+        mg = bld.create_mesh(base, x, y, z, snap.npart_esp, nl_mesh=max(g.nl, 1))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.mesh_steps):
+            mg = bld.create_mesh(base, x, y, z, snap.npart_esp, nl_mesh=max(g.nl, 1))
+        torch.cuda.synchronize()
+        tm = (time.perf_counter() - t0) / args.mesh_steps
+        out["create_mesh"] = {"seconds": tm, "levels": mg.nl, "npatch": mg.npatch.tolist(),
+                              "params": "asohf.dat mesh block (REFINE_THR 3, FRAC 0.05, MIN_PATCHSIZE 14, "
+                                        "BOR 4, BORAMR 0, BOR_OVLP 0), NAMRX..Z 32, NPALEV 10000",
+                              "routine": "CREATE_MESH (grids.f:2-707)"}
+        log(f"[bench] create_mesh: {tm:.3f} s, npatch {mg.npatch.tolist()}")
 
     if args.masks_steps > 0 and g.pcx is not None:
         # the grid masks and the FW2 density dump (asohf.f:515-545)

@@ -243,6 +243,38 @@ asohf_status asohf_hip_write_density_dump(asohf_ctx *ctx, const char *path, cons
                                           const int32_t *cr0amr, const float *u11, const int32_t *cr0amr11,
                                           const int32_t *solap, const asohf_opts *opts);
 
+/* Parameters of the mesh builder: asohf.dat's mesh block (asohf.f:196-212)
+ * and the reference build's compile-time bounds (asohf_parameters.dat). */
+typedef struct asohf_mesh_params {
+  int32_t nl_mesh;        /* NL_MESH: levels to build (1..9)                     */
+  int32_t refine_thr;     /* REFINE_THR                                          */
+  int32_t min_patchsize;  /* MIN_PATCHSIZE (child cells)                         */
+  float frac_refinable;   /* MINFRAC_REFINABLE                                   */
+  int32_t bor, boramr;    /* BOR, BORAMR                                         */
+  int32_t bor_ovlp;       /* BOR_OVLP                                            */
+  int32_t namr[3];        /* NAMRX, NAMRY, NAMRZ: largest patch, in cells        */
+  int32_t npalev;         /* NPALEV: most patches in all                         */
+  int32_t mpapolev[9];    /* MPAPOLEV(1:NLEVELS): most patches per level         */
+} asohf_mesh_params;
+
+/* The mesh builder CREATE_MESH (grids.f:2-707), called at asohf.f:459-463.
+ * base: the base grid (nx..nz, lado0, dx..dz = COMMON /ESPACIADO/, radx..z
+ * or NULL for MALLA's); its patch fields are ignored.  rx, ry, rz: device
+ * FP32 [n], particles in reference order with npart_esp (host int32 [9])
+ * giving the species blocks (PLEV).  Outputs (host): *nl_mesh (levels that
+ * received patches), npatch[10] (NPATCH(0:9)), and for the P patches
+ * (P <= cap; ASOHF_ERR_CAPACITY otherwise, with npatch set) PATCHNX..Z,
+ * PATCHX..Z, PARE and PATCHRX..Z, numbered as the reference numbers them.
+ * The particle counts run on the GPU; the patch growth, which each new
+ * patch changes for the next, runs on the host as the reference's
+ * sequential loops.  Uses opts->stream; synchronises. */
+asohf_status asohf_hip_create_mesh(asohf_ctx *ctx, const asohf_geom *base, int64_t n, const float *rx,
+                                   const float *ry, const float *rz, const int32_t *npart_esp,
+                                   const asohf_mesh_params *params, int32_t *nl_mesh, int32_t *npatch,
+                                   int32_t *patchnx, int32_t *patchny, int32_t *patchnz, int32_t *patchx,
+                                   int32_t *patchy, int32_t *patchz, int32_t *pare, float *patchrx, float *patchry,
+                                   float *patchrz, int64_t cap, const asohf_opts *opts);
+
 /* Frees the device workspaces the context keeps between calls (binned
  * entries, work items, block tables, emission lists, split tiles, the
  * adjoint's sort buffers, the captured graph); the geometry plan stays, and

@@ -10,7 +10,8 @@ module, and only as the checker or the timed CPU baseline.  The product path
     (built by `make ref` where the reference tree exists); run_ref_species()
     and run_ref_sort_cells() call SORT_DM_PARTICLES_LOCALDENSITY and
     SORT_CELLS_AMR through the same driver, run_ref_dump() the grid masks
-    and the density dump of asohf.f:515-545.
+    and the density dump of asohf.f:515-545, run_ref_create_mesh() the mesh
+    builder CREATE_MESH.
 """
 from __future__ import annotations
 
@@ -484,3 +485,50 @@ def run_ref_dump(snap) -> bytes:
                        preexec_fn=big_stack)
         with open(fout, "rb") as f:
             return f.read()
+
+
+MESH_DEFAULTS = dict(refine_thr=3, min_patchsize=14, bor=4, boramr=0, bor_ovlp=0, frac_refinable=0.05)
+
+
+def run_ref_create_mesh(x, y, z, m, npart_esp, nx: int, lado0: float, nl_mesh: int, mpapolev=None, **params):
+    """The reference's CREATE_MESH (grids.f:2-707) on particles in reference
+    order (species contiguous): a dict with nl_mesh, npatch[0:NLEVELS+1] and
+    the patch arrays (pnx.., px.., pare, prx..)."""
+    import resource
+    prm = dict(MESH_DEFAULTS, **params)
+    x, y, z, m = (np.ascontiguousarray(a, np.float32) for a in (x, y, z, m))
+    n = len(x)
+    mp = np.full(9, 10000, np.int32) if mpapolev is None else np.asarray(mpapolev, np.int32)
+
+    def big_stack():
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            np.array([6, nx, nx, nx, nl_mesh, n, 2, 0], np.int32).tofile(f)
+            np.zeros(nl_mesh + 1, np.int32).tofile(f)
+            np.asarray(npart_esp, np.int32).tofile(f)
+            np.array([lado0, 1.0, 1.0, 1.0], np.float32).tofile(f)
+            for a in (x, y, z, m):
+                a.tofile(f)
+            np.array([prm["refine_thr"], prm["min_patchsize"], prm["bor"], prm["boramr"], prm["bor_ovlp"]],
+                     np.int32).tofile(f)
+            np.array([prm["frac_refinable"]], np.float32).tofile(f)
+            mp.tofile(f)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([REF_BIN, fin, fout], check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       env=env, preexec_fn=big_stack)
+        raw = np.fromfile(fout, np.uint8)
+    i32 = lambda a, b: np.frombuffer(raw[a:b].tobytes(), np.int32)
+    nlm = int(i32(0, 4)[0])
+    npatch = i32(4, 44).copy()
+    P = int(npatch.sum())
+    out = {"nl_mesh": nlm, "npatch": npatch}
+    pos = 44
+    for k in ("pnx", "pny", "pnz", "px", "py", "pz", "pare"):
+        out[k] = i32(pos, pos + 4 * P).copy()
+        pos += 4 * P
+    for k in ("prx", "pry", "prz"):
+        out[k] = np.frombuffer(raw[pos:pos + 4 * P].tobytes(), np.float32).copy()
+        pos += 4 * P
+    return out

@@ -22,6 +22,13 @@
 *               sequential file.  Extra input after the particles: int32
 *               patchx(P), patchy(P), patchz(P), pare(P).  Needs a large
 *               stack (VEINSGRID's VECINO is NPALEV x NPALEV).
+*       mode 6: SUBROUTINE CREATE_MESH (grids.f:2-707) with NL_MESH =
+*               hdr(5) on the particles (P = 0, species from npart_esp).
+*               Extra input after the particles: int32 refine_thr,
+*               min_patchsize, bor, boramr, bor_ovlp; real*4
+*               frac_refinable; int32 mpapolev(NLEVELS).  Output: int32
+*               nl_mesh, npatch(0:NLEVELS), then for the P patches int32
+*               pnx, pny, pnz, px, py, pz, pare and real*4 prx, pry, prz.
 *     The grid constants come from the reference's MALLA (grids.f:2509)
 *     called with LADO = LADO0 - LADO0/NX as asohf.f:323 does.
 *
@@ -66,6 +73,8 @@
       INTEGER CR0AMR(NMAX,NMAY,NMAZ)
       INTEGER CR0AMR11(NAMRX,NAMRY,NAMRZ,NPALEV)
       INTEGER IR
+      INTEGER RTHR,MINPS,BOR,BORAMR,BOVLP,MPAPOLEV(NLEVELS),NLM
+      REAL*4 FRACR
       INTEGER, ALLOCATABLE :: DDDX(:),DDDY(:),DDDZ(:),DDDP(:)
 
       CALL GET_COMMAND_ARGUMENT(1,FIN)
@@ -141,6 +150,11 @@
        READ(10) (RYPA(I),I=1,N)
        READ(10) (RZPA(I),I=1,N)
        READ(10) (MASAP(I),I=1,N)
+       IF (MODE.EQ.6) THEN
+        READ(10) RTHR,MINPS,BOR,BORAMR,BOVLP
+        READ(10) FRACR
+        READ(10) MPAPOLEV
+       END IF
        IF (MODE.EQ.5) THEN
         READ(10) (PATCHX(I),I=1,NPT)
         READ(10) (PATCHY(I),I=1,NPT)
@@ -155,6 +169,32 @@
       T=0.0
       ZETA=0.0
 
+      IF (MODE.EQ.6) THEN
+       NLM=NL
+       NPATCH=0
+       CALL CREATE_MESH(0,NX,NY,NZ,NLM,NPATCH,PARE,
+     &           PATCHNX,PATCHNY,PATCHNZ,PATCHX,PATCHY,PATCHZ,
+     &           PATCHRX,PATCHRY,PATCHRZ,N,N,0,LADO0,T,
+     &           ZETA,RTHR,MINPS,FRACR,BOR,
+     &           BORAMR,BOVLP,NPART_ESP,0,MPAPOLEV)
+       NPT=SUM(NPATCH(0:NLEVELS))
+       OPEN(11,FILE=FOUT,ACCESS='STREAM',FORM='UNFORMATTED',
+     &      STATUS='REPLACE',ACTION='WRITE')
+       WRITE(11) NLM
+       WRITE(11) NPATCH
+       WRITE(11) (PATCHNX(I),I=1,NPT)
+       WRITE(11) (PATCHNY(I),I=1,NPT)
+       WRITE(11) (PATCHNZ(I),I=1,NPT)
+       WRITE(11) (PATCHX(I),I=1,NPT)
+       WRITE(11) (PATCHY(I),I=1,NPT)
+       WRITE(11) (PATCHZ(I),I=1,NPT)
+       WRITE(11) (PARE(I),I=1,NPT)
+       WRITE(11) (PATCHRX(I),I=1,NPT)
+       WRITE(11) (PATCHRY(I),I=1,NPT)
+       WRITE(11) (PATCHRZ(I),I=1,NPT)
+       CLOSE(11)
+       STOP
+      END IF
       IF (MODE.EQ.3) THEN
        ALLOCATE(U2DM(PARTI),U3DM(PARTI),U4DM(PARTI),ORIPA(PARTI))
        DO I=1,N
