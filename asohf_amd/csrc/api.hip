@@ -1299,6 +1299,7 @@ asohf_status asohf_hip_peak_candidates(asohf_ctx* ctx, const asohf_geom* geom, i
     for (long long p = 0; p < before + np; ++p) {
       const int a = geom->patchnx[p], b = geom->patchny[p], c = geom->patchnz[p];
       if (a < 1 || b < 1 || c < 1) return fail(ctx, ASOHF_ERR_INVALID, "patch sizes must be >= 1");
+      if ((long long)a * b * c > (1ll << 30)) return fail(ctx, ASOHF_ERR_CAPACITY, "patch too large");
       if (padded && (a > opts->u11_ld[0] || b > opts->u11_ld[1] || c > opts->u11_ld[2]))
         return fail(ctx, ASOHF_ERR_INVALID, "a patch exceeds the padded U11 dimensions");
       const long long cells = (long long)a * b * c;
