@@ -43,6 +43,25 @@ def test_null_context_is_rejected():
     assert L.asohf_hip_density(None, None, 0, None, None, None, None, None, None, None, None, None) == -1
     assert L.asohf_hip_get_stats(None, None) == -1
     assert L.asohf_hip_release_workspaces(None) == -1
+    assert L.asohf_hip_species(None, None, None, 0.0, 0.0, 0, None, None, None, None, None, None, None,
+                               None) == -1
+    assert L.asohf_hip_peak_candidates(None, None, 0, None, None, None, 0.0, None, None, None, None, None, 0,
+                                       None, None) == -1
+    assert L.asohf_hip_refinement_masks(None, None, None, None, None, None, None, None, None) == -1
+    assert L.asohf_hip_overlap_mask(None, None, None, None) == -1
+    assert L.asohf_hip_write_density_dump(None, b"/nonexistent", None, None, None, None, None, None, None) == -1
+    assert L.asohf_hip_create_mesh(None, None, 0, None, None, None, None, None, None, None, None, None, None,
+                                   None, None, None, None, None, None, None, 0, None) == -1
+
+
+def test_mesh_params_layout_matches_header():
+    # asohf_mesh_params: 7 scalars, namr[3], npalev, mpapolev[9] (4-byte fields, no padding)
+    assert C.sizeof(_lib.MeshParams) == 4 * (7 + 3 + 1 + 9)
+    txt = open(os.path.join(ROOT, "include", "asohf_hip.h")).read()
+    body = txt[txt.index("typedef struct asohf_mesh_params"):txt.index("} asohf_mesh_params;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)          # comments out
+    fields = re.findall(r"\b(\w+)(?:\[\d\])?[,;]", body)
+    assert fields == [f for f, _ in _lib.MeshParams._fields_], fields
 
 
 def test_synthetic_hierarchy_obeys_mesh_rules():
