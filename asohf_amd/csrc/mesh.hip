@@ -19,8 +19,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "dev.h"
@@ -331,24 +333,35 @@ std::string mesh_next_level(int irpa, const std::vector<MeshPatchHost>& all, int
       }
     }
   }
-  // ---- growth inside every parent (grids.f:449-588) ----
+  // ---- growth inside every parent (grids.f:449-588).  Parents are
+  // independent (each touches only its own cells, as the reference's
+  // OpenMP loop over IPARE assumes), so they run on host threads; their
+  // patches are concatenated in parent order, as the sequential loop
+  // produces them.  The refinable count is kept up to date from the boxes
+  // each step overwrites, instead of recounting the parent's cells. ----
   const int npalev3 = (pr.namr[0] / 5) * (pr.namr[0] / 5) * (pr.namr[0] / 5) + 1;
   struct Cand {
     MeshPatchHost m;
     int val, parent, rank;
   };
-  std::vector<Cand> cands;
-  for (int k = 0; k < np; ++k) {
+  std::vector<std::vector<Cand>> per(np);
+  auto grow_parent = [&](int k) {
     const MeshPatchHost& P = all[(size_t)(first + k)];
     Grid3 c11{cr.data() + off[k], {P.n[0], P.n[1], P.n[2]}};
     Grid3 k11{conta.data() + off[k], {P.n[0], P.n[1], P.n[2]}};
     const long long cells = (long long)P.n[0] * P.n[1] * P.n[2];
-    auto refinable = [&]() {
-      long long c = 0;
-      for (long long q = 0; q < cells; ++q) c += c11.v[q] >= pr.refine_thr;
-      return c;
+    long long refine_count = 0;
+    for (long long q = 0; q < cells; ++q) refine_count += c11.v[q] >= pr.refine_thr;
+    // overwrite a box of c11 with v, keeping refine_count = COUNT(c11 >= thr)
+    auto set_c11 = [&](const int bx[6], int v) {
+      for (int kk = bx[4]; kk <= bx[5]; ++kk)
+        for (int jj = bx[2]; jj <= bx[3]; ++jj)
+          for (int ii = bx[0]; ii <= bx[1]; ++ii) {
+            int& cv = c11.at(ii, jj, kk);
+            refine_count += (long long)(v >= pr.refine_thr) - (long long)(cv >= pr.refine_thr);
+            cv = v;
+          }
     };
-    long long refine_count = refinable();
     int ipatch = 0;
     const int lo[3] = {pr.boramr + 1, pr.boramr + 1, pr.boramr + 1};
     const int hi[3] = {P.n[0] - pr.boramr, P.n[1] - pr.boramr, P.n[2] - pr.boramr};
@@ -361,38 +374,50 @@ std::string mesh_next_level(int irpa, const std::vector<MeshPatchHost>& all, int
       const int seed[3] = {(int)(best % P.n[0]) + 1, (int)((best / P.n[0]) % P.n[1]) + 1,
                            (int)(best / ((long long)P.n[0] * P.n[1])) + 1};
       const int basint = c11.v[best];
-      int b[6], n[3];
-      for (int a = 0; a < 3; ++a) {
-        b[2 * a] = std::max(seed[a] - 1, lo[a]);
-        b[2 * a + 1] = std::min(seed[a] + 1, hi[a]);
-        n[a] = 2 * (b[2 * a + 1] - b[2 * a] + 1);
+      int bb[6], n[3];
+      for (int ax = 0; ax < 3; ++ax) {
+        bb[2 * ax] = std::max(seed[ax] - 1, lo[ax]);
+        bb[2 * ax + 1] = std::min(seed[ax] + 1, hi[ax]);
+        n[ax] = 2 * (bb[2 * ax + 1] - bb[2 * ax] + 1);
       }
-      grow(k11, b, n, lo, hi, pr);
+      grow(k11, bb, n, lo, hi, pr);
       if (std::min(n[0], std::min(n[1], n[2])) < pr.min_patchsize) {
-        fill_box(c11, b, 0);
-        fill_box(k11, b, 0);
+        set_c11(bb, 0);
+        fill_box(k11, bb, 0);
       } else {
         ++ipatch;
         int o[6];
-        bis_box(b, seed, pr.bor_ovlp, o);
+        bis_box(bb, seed, pr.bor_ovlp, o);
         fill_box(k11, o, 0);
-        fill_box(c11, o, -1);
+        set_c11(o, -1);
         Cand cd;
-        for (int a = 0; a < 3; ++a) {
-          cd.m.n[a] = n[a];
-          cd.m.pc[a] = b[2 * a];
+        for (int ax = 0; ax < 3; ++ax) {
+          cd.m.n[ax] = n[ax];
+          cd.m.pc[ax] = bb[2 * ax];
           // the parent's cell size: PATCHRX(IPARE)+(I1-1.5)*DXPA
-          cd.m.pr[a] = P.pr[a] + ((float)b[2 * a] - 1.5f) * d[a];
+          cd.m.pr[ax] = P.pr[ax] + ((float)bb[2 * ax] - 1.5f) * d[ax];
         }
         cd.m.pare = first + k + 1;
         cd.val = basint;
         cd.parent = k;
         cd.rank = ipatch;
-        cands.push_back(cd);
+        per[(size_t)k].push_back(cd);
       }
-      refine_count = refinable();
     }
+  };
+  {
+    const int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::atomic<int> next{0};
+    auto worker = [&]() {
+      for (int k = next.fetch_add(1); k < np; k = next.fetch_add(1)) grow_parent(k);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < std::min(nthreads, np); ++t) pool.emplace_back(worker);
+    worker();
+    for (std::thread& th : pool) th.join();
   }
+  std::vector<Cand> cands;
+  for (int k = 0; k < np; ++k) cands.insert(cands.end(), per[(size_t)k].begin(), per[(size_t)k].end());
   // ---- assembly: MAXLOC over LVAL(I, IPARE), i.e. decreasing seed count,
   // then parent, then order within the parent (grids.f:593-621) ----
   std::stable_sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.val > b.val; });
