@@ -625,6 +625,124 @@ __device__ __forceinline__ void deposit27(typename AccT<RED>::T* base, float m, 
   }
 }
 
+// the value of lane l^1 (l^2) of the wave, through a quad permutation
+__device__ __forceinline__ unsigned from_xor1(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+}
+__device__ __forceinline__ unsigned from_xor2(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+}
+// the value of lane l+4 (l-4) of the same row of 16 lanes
+__device__ __forceinline__ unsigned from_plus4(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x104, 0xF, 0xF, false);   // row_shl:4
+}
+__device__ __forceinline__ unsigned from_minus4(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);   // row_shr:4
+}
+__device__ __forceinline__ long long from_plus4(long long v) {
+  const unsigned lo = from_plus4((unsigned)v), hi = from_plus4((unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ long long from_xor1(long long v) {
+  const unsigned lo = from_xor1((unsigned)v), hi = from_xor1((unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ long long from_xor2(long long v) {
+  const unsigned lo = from_xor2((unsigned)v), hi = from_xor2((unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Lanes of a merge step needed before it pays (each merged pair saves one
+// lane's 27 LDS adds and costs every lane three instructions per term).
+constexpr int kMergeMin = 4;
+
+// The deterministic TSC deposit with lanes merged first.  Clustered
+// particles put many lanes of a wave on the same stencil, and the LDS
+// serialises adds to one address; exact integer terms can be summed in
+// registers instead, with the same result.  Three butterfly steps: lane l
+// takes lane l^1's terms when both stencils start at the same cell (l even),
+// then lane l^2's (bit 1 of l clear), then lane l^4's (bit 2 clear).  Every
+// non-centre TSC term is at most 0.5 * 0.75 * 0.75 of Q(m) < 2^31 (one factor
+// is an outer weight), so four of them sum below 2^32 and the third step adds
+// in 64 bits; the centre remainder, negative for a light enough particle,
+// merges as int64.  Every lane of the wave must be active (the steps read
+// other lanes); ok = the lane holds an entry, corner = its stencil's first
+// cell in the tile.
+template <int HX, int HY>
+__device__ __forceinline__ void deposit27_merged(long long* acc, int corner, bool ok, float m, const float wx[3],
+                                                 const float wy[3], const float wz[3], float scale) {
+  const unsigned lane = __lane_id();
+  // lanes without an entry get keys that match no other lane
+  const unsigned key0 = ok ? (unsigned)corner : (0x80000000u | lane);
+  const bool m1 = from_xor1(key0) == key0;
+  if (__popcll(__ballot(m1)) < kMergeMin) {
+    if (ok) deposit27<0, HX, HY>(acc + corner, m, wx, wy, wz, scale);
+    return;
+  }
+  const bool keep1 = m1 && !(lane & 1u);
+  const bool alive1 = ok && !(m1 && (lane & 1u));
+  const unsigned key1 = alive1 ? (unsigned)corner : (0x80000000u | lane);
+  const bool m2 = from_xor2(key1) == key1;
+  const bool step2 = __popcll(__ballot(m2)) >= kMergeMin;
+  const bool keep2 = step2 && m2 && !(lane & 2u);
+  const bool alive2 = alive1 && !(step2 && m2 && (lane & 2u));
+  // lane l^4 stays in the row of 16 lanes, so one row shift reaches it.  Both
+  // shifts run on every lane and the lane picks its value afterwards: under a
+  // branch, half the lanes would be inactive and their values unreadable.
+  const unsigned key2 = alive2 ? (unsigned)corner : (0x80000000u | lane);
+  const unsigned k_minus = from_minus4(key2), k_plus = from_plus4(key2);
+  const unsigned pk = (lane & 4u) ? k_minus : k_plus;
+  const bool m3 = pk == key2;
+  const bool step3 = __popcll(__ballot(m3)) >= kMergeMin;
+  const bool keep3 = step3 && m3 && !(lane & 4u);
+  const bool alive = alive2 && !(step3 && m3 && (lane & 4u));
+
+  const float ms = m * scale;
+  float mx[3], mxy[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) mx[a] = ms * wx[a];
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) mxy[b][a] = mx[a] * wy[b];
+  long long* base = acc + corner;
+  unsigned rest = 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (a == 1 && b == 1 && c == 1) continue;
+        unsigned q = __float2uint_rn(mxy[b][a] * wz[c]);
+        rest += q;
+        const unsigned p1 = from_xor1(q);
+        q = keep1 ? q + p1 : q;
+        if (step2) {
+          const unsigned p2 = from_xor2(q);
+          q = keep2 ? q + p2 : q;
+        }
+        unsigned long long q3 = q;
+        if (step3) {
+          const unsigned p3 = from_plus4(q);
+          q3 = keep3 ? q3 + p3 : q3;
+        }
+        if (alive) lds_add(base + (a + HX * (b + HY * c)), (long long)q3);
+      }
+  long long cen = (long long)__float2uint_rn(ms) - (long long)rest;
+  const long long c1 = from_xor1(cen);
+  cen = keep1 ? cen + c1 : cen;
+  if (step2) {
+    const long long c2 = from_xor2(cen);
+    cen = keep2 ? cen + c2 : cen;
+  }
+  if (step3) {
+    const long long c3 = from_plus4(cen);
+    cen = keep3 ? cen + c3 : cen;
+  }
+  if (alive) lds_add(base + (1 + HX * (1 + HY * 1)), cen);
+}
+
 // base-grid cell centre RAD(i), i = 0..n+1: MALLA's own FP32 recurrence
 // (RAD(i) = RAD(1)+(i-1)*C, ghost RAD(n+1) = RAD(n)+C) when the plan found the
 // table to be exactly that, a table read otherwise
@@ -706,10 +824,12 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
       // 64 consecutive entries (1 KB, every entry fetched once).  Two entries
       // per pass, in two registers refilled right after use (register copies
       // would wait on the loads); the prefetch indices are clamped into the
-      // item instead of tested, so every pass issues the same loads.  An
-      // empty item reads nothing: no lane enters the loop.
+      // item instead of tested, so every pass issues the same loads, and the
+      // loop runs the same number of passes on every lane (ok = the lane holds
+      // an entry), which the lane merging of the TSC deposit needs.  An empty
+      // item reads nothing: no pass runs.
       const long long len = it.end - it.begin;
-      auto narrow = [&](const int4 v) {
+      auto narrow = [&](const int4 v, const bool ok) {
         int ci[3];
         float wx[3], wy[3], wz[3], m;
         if (MODE == 2) {
@@ -743,8 +863,11 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
         const int ly = clampi(centre_local(ci[1], j0, ey, G.n[1], periodic), -1, TY);
         const int lz = clampi(centre_local(ci[2], k0c, ez, G.n[2], periodic), -1, TZ);
         // corner (-1,-1,-1) of the stencil; halo coordinates are local + 2
-        Acc* base = acc + ((lx + 1) + HX * ((ly + 1) + HY * (lz + 1)));
-        deposit27<RED, HX, HY>(base, m, wx, wy, wz, scale);
+        const int corner = (lx + 1) + HX * ((ly + 1) + HY * (lz + 1));
+        if (MODE == 2 && RED == 0)
+          deposit27_merged<HX, HY>(reinterpret_cast<long long*>(acc), corner, ok, m, wx, wy, wz, scale);
+        else if (ok)
+          deposit27<RED, HX, HY>(acc + corner, m, wx, wy, wz, scale);
       };
       const long long last = it.end - 1;
       int4 va = make_int4(0, 0, 0, 0), vb = va;
@@ -753,11 +876,11 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
         va = D.entries[min(e0, last)];
         vb = D.entries[min(e0 + nthr, last)];
       }
-      for (long long e = e0; e < it.end; e += 2 * nthr) {
-        narrow(va);
-        va = D.entries[min(e + 2 * nthr, last)];
-        if (e + nthr < it.end) narrow(vb);
-        vb = D.entries[min(e + 3 * nthr, last)];
+      for (long long o = 0; o < len; o += 2 * nthr) {
+        narrow(va, o + tid < len);
+        va = D.entries[min(e0 + o + 2 * nthr, last)];
+        narrow(vb, o + nthr + tid < len);
+        vb = D.entries[min(e0 + o + 3 * nthr, last)];
       }
     } else if (MODE == 1) {
       // ---- wide table-mode entries: one wave per entry.  The entry's box
