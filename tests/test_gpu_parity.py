@@ -116,6 +116,59 @@ def test_deterministic_bit_identical_across_runs_and_tiles(bld, mode):
         bld.set_tune(**old)
 
 
+def test_clustered_stencils_merge_exactly(bld):
+    """Clumps of particles inside single cells on both sides of tile borders
+    (cells 7|8, 15|16, 23|24, 31|32 and the periodic wrap), many thousands per
+    cell: most lanes of a wave share one stencil, among them stencils that start
+    at a tile halo's first cell (centre in the cell just below the tile), so the
+    deposit sums lanes in registers in all its merge steps.  Masses span 1 to
+    1e-7, so some centre remainders are negative.  Every particle deposits
+    exactly its quantised mass Q(m) = rn(m * scale) (scale = 2^(31-e), e the
+    binary exponent of the largest mass), so the base-grid mass equals
+    sum(Q) / scale exactly; the fields are bit-identical under a shuffled
+    particle order and across tile shapes, and they match the FP64 oracle."""
+    src = synth.make_snapshot(synth.CONFIGS[1], n_side=16, nx=32, patches_per_level=(3, 2), seed=5)
+    g = src.geom
+    rng = np.random.default_rng(11)
+    dx = float(g.dx)
+    x0 = -0.5 * g.nx * dx
+    cells = np.array([7, 8, 15, 16, 23, 24, 31, 32])   # 1-based base cells
+    pos, mass = [], []
+    for _ in range(40):
+        c = rng.choice(cells, 3)
+        n = int(rng.integers(500, 4000))
+        pos.append(x0 + (c - 0.5) * dx + rng.uniform(-0.45, 0.45, (n, 3)) * dx)
+        mass.append(np.full(n, rng.choice([1.0, 1e-3, 1e-7]), np.float32))
+    pos = np.concatenate(pos).astype(np.float32)
+    m = np.concatenate(mass)
+    n = len(m)
+    esp = np.zeros(9, np.int32)
+    esp[0] = n
+    snap = synth.Snapshot(pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy(), m, esp, g)
+    print(f"case clustered stencils: {n} particles", flush=True)
+    ref = run(bld, snap, "tsc")
+    scale = 2.0 ** (31 - int(np.frexp(np.float32(m.max()))[1]))
+    q = np.rint(m.astype(np.float64) * scale)   # m * scale is exact; rint rounds half to even
+    assert ref[2][0] == q.sum() / scale
+    tot = float(m.astype(np.float64).sum())
+    assert abs(ref[2][0] - tot) <= 0.5 * n / scale
+    perm = rng.permutation(n)
+    shuf = synth.Snapshot(snap.x[perm], snap.y[perm], snap.z[perm], m[perm], esp, g)
+    got = run(bld, shuf, "tsc")
+    assert all(np.array_equal(a, b) for a, b in zip(ref, got))
+    old = bld.get_tune()
+    try:
+        for tile in ((8, 8, 8), (16, 16, 16)):
+            bld.set_tune(tile=tile, block=512, chunk=8192, waves_per_cu=16)
+            got = run(bld, snap, "tsc")
+            assert all(np.array_equal(a, b) for a, b in zip(ref, got)), tile
+    finally:
+        bld.set_tune(**old)
+    d1, d11, _ = O.density64(snap, "B")
+    assert rel_err_above_mean(ref[0], d1)[0] <= 1e-5
+    assert rel_err_above_mean(ref[1], d11)[0] <= 1e-5
+
+
 @pytest.mark.parametrize("mode", ["table", "tsc"])
 def test_plain_form_vs_oracle(bld, golden, mode):
     snap, _ = golden
