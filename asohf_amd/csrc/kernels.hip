@@ -679,13 +679,15 @@ __device__ __forceinline__ void deposit27_merged(long long* acc, int corner, boo
     if (ok) deposit27<0, HX, HY>(acc + corner, m, wx, wy, wz, scale);
     return;
   }
-  const bool keep1 = m1 && !(lane & 1u);
-  const bool alive1 = ok && !(m1 && (lane & 1u));
+  // a lane absorbed by a step sends its value and every lane adds what it
+  // receives: keepers receive their partner's, everyone else receives 0
+  const bool send1 = m1 && (lane & 1u);
+  const bool alive1 = ok && !send1;
   const unsigned key1 = alive1 ? (unsigned)corner : (0x80000000u | lane);
   const bool m2 = from_xor2(key1) == key1;
   const bool step2 = __popcll(__ballot(m2)) >= kMergeMin;
-  const bool keep2 = step2 && m2 && !(lane & 2u);
-  const bool alive2 = alive1 && !(step2 && m2 && (lane & 2u));
+  const bool send2 = step2 && m2 && (lane & 2u);
+  const bool alive2 = alive1 && !send2;
   // lane l^4 stays in the row of 16 lanes, so one row shift reaches it.  Both
   // shifts run on every lane and the lane picks its value afterwards: under a
   // branch, half the lanes would be inactive and their values unreadable.
@@ -694,8 +696,8 @@ __device__ __forceinline__ void deposit27_merged(long long* acc, int corner, boo
   const unsigned pk = (lane & 4u) ? k_minus : k_plus;
   const bool m3 = pk == key2;
   const bool step3 = __popcll(__ballot(m3)) >= kMergeMin;
-  const bool keep3 = step3 && m3 && !(lane & 4u);
-  const bool alive = alive2 && !(step3 && m3 && (lane & 4u));
+  const bool send3 = step3 && m3 && (lane & 4u);
+  const bool alive = alive2 && !send3;
 
   const float ms = m * scale;
   float mx[3], mxy[3][3];
@@ -716,30 +718,16 @@ __device__ __forceinline__ void deposit27_merged(long long* acc, int corner, boo
         if (a == 1 && b == 1 && c == 1) continue;
         unsigned q = __float2uint_rn(mxy[b][a] * wz[c]);
         rest += q;
-        const unsigned p1 = from_xor1(q);
-        q = keep1 ? q + p1 : q;
-        if (step2) {
-          const unsigned p2 = from_xor2(q);
-          q = keep2 ? q + p2 : q;
-        }
+        q += from_xor1(send1 ? q : 0u);
+        if (step2) q += from_xor2(send2 ? q : 0u);
         unsigned long long q3 = q;
-        if (step3) {
-          const unsigned p3 = from_plus4(q);
-          q3 = keep3 ? q3 + p3 : q3;
-        }
+        if (step3) q3 += from_plus4(send3 ? q : 0u);
         if (alive) lds_add(base + (a + HX * (b + HY * c)), (long long)q3);
       }
   long long cen = (long long)__float2uint_rn(ms) - (long long)rest;
-  const long long c1 = from_xor1(cen);
-  cen = keep1 ? cen + c1 : cen;
-  if (step2) {
-    const long long c2 = from_xor2(cen);
-    cen = keep2 ? cen + c2 : cen;
-  }
-  if (step3) {
-    const long long c3 = from_plus4(cen);
-    cen = keep3 ? cen + c3 : cen;
-  }
+  cen += from_xor1(send1 ? cen : 0ll);
+  if (step2) cen += from_xor2(send2 ? cen : 0ll);
+  if (step3) cen += from_plus4(send3 ? cen : 0ll);
   if (alive) lds_add(base + (1 + HX * (1 + HY * 1)), cen);
 }
 
