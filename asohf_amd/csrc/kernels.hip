@@ -643,6 +643,9 @@ __device__ __forceinline__ long long from_plus4(long long v) {
   const unsigned lo = from_plus4((unsigned)v), hi = from_plus4((unsigned)((unsigned long long)v >> 32));
   return (long long)(((unsigned long long)hi << 32) | lo);
 }
+__device__ __forceinline__ float from_xor1(float v) { return __uint_as_float(from_xor1(__float_as_uint(v))); }
+__device__ __forceinline__ float from_xor2(float v) { return __uint_as_float(from_xor2(__float_as_uint(v))); }
+__device__ __forceinline__ float from_plus4(float v) { return __uint_as_float(from_plus4(__float_as_uint(v))); }
 __device__ __forceinline__ long long from_xor1(long long v) {
   const unsigned lo = from_xor1((unsigned)v), hi = from_xor1((unsigned)((unsigned long long)v >> 32));
   return (long long)(((unsigned long long)hi << 32) | lo);
@@ -653,13 +656,15 @@ __device__ __forceinline__ long long from_xor2(long long v) {
 }
 
 // Lanes of a merge step needed before it pays (each merged pair saves one
-// lane's 27 LDS adds and costs every lane three instructions per term).
+// lane's 27 LDS adds and costs every lane two instructions per term).
 constexpr int kMergeMin = 4;
 
-// The deterministic TSC deposit with lanes merged first.  Clustered
-// particles put many lanes of a wave on the same stencil, and the LDS
-// serialises adds to one address; exact integer terms can be summed in
-// registers instead, with the same result.  Three butterfly steps: lane l
+// The TSC deposit with lanes merged first.  Clustered particles put many
+// lanes of a wave on the same stencil, and the LDS serialises adds to one
+// address; the terms can be summed in registers instead.  Deterministic mode:
+// the terms are exact integers, so the result is the same.  Atomic mode: an
+// FP32 sum in another order, as any order of its LDS atomics is.  Three
+// butterfly steps: lane l
 // takes lane l^1's terms when both stencils start at the same cell (l even),
 // then lane l^2's (bit 1 of l clear), then lane l^4's (bit 2 clear).  Every
 // non-centre TSC term is at most 0.5 * 0.75 * 0.75 of Q(m) < 2^31 (one factor
@@ -668,15 +673,16 @@ constexpr int kMergeMin = 4;
 // merges as int64.  Every lane of the wave must be active (the steps read
 // other lanes); ok = the lane holds an entry, corner = its stencil's first
 // cell in the tile.
-template <int HX, int HY>
-__device__ __forceinline__ void deposit27_merged(long long* acc, int corner, bool ok, float m, const float wx[3],
-                                                 const float wy[3], const float wz[3], float scale) {
+template <int RED, int HX, int HY>
+__device__ __forceinline__ void deposit27_merged(typename AccT<RED>::T* acc, int corner, bool ok, float m,
+                                                 const float wx[3], const float wy[3], const float wz[3],
+                                                 float scale) {
   const unsigned lane = __lane_id();
   // lanes without an entry get keys that match no other lane
   const unsigned key0 = ok ? (unsigned)corner : (0x80000000u | lane);
   const bool m1 = from_xor1(key0) == key0;
   if (__popcll(__ballot(m1)) < kMergeMin) {
-    if (ok) deposit27<0, HX, HY>(acc + corner, m, wx, wy, wz, scale);
+    if (ok) deposit27<RED, HX, HY>(acc + corner, m, wx, wy, wz, scale);
     return;
   }
   // a lane absorbed by a step sends its value and every lane adds what it
@@ -699,7 +705,7 @@ __device__ __forceinline__ void deposit27_merged(long long* acc, int corner, boo
   const bool send3 = step3 && m3 && (lane & 4u);
   const bool alive = alive2 && !send3;
 
-  const float ms = m * scale;
+  const float ms = RED == 0 ? m * scale : m;
   float mx[3], mxy[3][3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) mx[a] = ms * wx[a];
@@ -707,7 +713,23 @@ __device__ __forceinline__ void deposit27_merged(long long* acc, int corner, boo
   for (int b = 0; b < 3; ++b)
 #pragma unroll
     for (int a = 0; a < 3; ++a) mxy[b][a] = mx[a] * wy[b];
-  long long* base = acc + corner;
+  if (RED == 1) {
+    float* fbase = reinterpret_cast<float*>(acc) + corner;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          float t = mxy[b][a] * wz[c];
+          t += from_xor1(send1 ? t : 0.f);
+          if (step2) t += from_xor2(send2 ? t : 0.f);
+          if (step3) t += from_plus4(send3 ? t : 0.f);
+          if (alive) lds_add(fbase + (a + HX * (b + HY * c)), t);
+        }
+    return;
+  }
+  long long* base = reinterpret_cast<long long*>(acc) + corner;
   unsigned rest = 0;
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -852,8 +874,8 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
         const int lz = clampi(centre_local(ci[2], k0c, ez, G.n[2], periodic), -1, TZ);
         // corner (-1,-1,-1) of the stencil; halo coordinates are local + 2
         const int corner = (lx + 1) + HX * ((ly + 1) + HY * (lz + 1));
-        if (MODE == 2 && RED == 0)
-          deposit27_merged<HX, HY>(reinterpret_cast<long long*>(acc), corner, ok, m, wx, wy, wz, scale);
+        if (MODE == 2)
+          deposit27_merged<RED, HX, HY>(acc, corner, ok, m, wx, wy, wz, scale);
         else if (ok)
           deposit27<RED, HX, HY>(acc + corner, m, wx, wy, wz, scale);
       };
