@@ -531,6 +531,19 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   D.hmask = (unsigned)(cap - 1);
   for (int a = 0; a < 3; ++a) { D.tile[a] = T[a]; D.tshift[a] = __builtin_ctz(T[a]); }
   D.mode = mode;
+  // lane merging of narrow deposits sums up to four non-centre terms in 32
+  // bits: TSC weight products are at most 0.5 * 0.75 * 0.75; the table's row
+  // 0 (centred at kmax) is checked here
+  {
+    const float* k0 = H.ktab.data() + D.kmax - 1;
+    float mx = 0.f;
+    bool nonneg = k0[0] >= 0.f && k0[1] >= 0.f && k0[2] >= 0.f;
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a)
+          if (!(a == 1 && b == 1 && c == 1)) mx = std::max(mx, k0[a] * k0[b] * k0[c]);
+    D.merge_narrow = mode == 2 || (nonneg && mx <= 0.45f) ? 1 : 0;
+  }
   // are the centre tables MALLA's recurrence?  (always, unless the caller
   // passed its own COMMON /GRID/ values that differ)
   D.rad_malla = 1;

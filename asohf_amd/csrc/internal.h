@@ -80,6 +80,8 @@ struct Plan {
   float fdm;
   int nested;                // 1: every level-l box lies inside a level-(l-1) box (FP32 check)
   int rad_malla;             // 1: RAD tables are exactly MALLA's recurrence (rad1 + (i-1)*radc)
+  int merge_narrow;          // 1: narrow deposits may merge lanes (every non-centre weight product
+                             // <= 0.45, so four terms of a mass below 2^31 sum below 2^32)
   float rad1[3];
   float radc[3];
 };

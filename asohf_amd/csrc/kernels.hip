@@ -659,20 +659,22 @@ __device__ __forceinline__ long long from_xor2(long long v) {
 // lane's 27 LDS adds and costs every lane two instructions per term).
 constexpr int kMergeMin = 4;
 
-// The TSC deposit with lanes merged first.  Clustered particles put many
-// lanes of a wave on the same stencil, and the LDS serialises adds to one
-// address; the terms can be summed in registers instead.  Deterministic mode:
-// the terms are exact integers, so the result is the same.  Atomic mode: an
-// FP32 sum in another order, as any order of its LDS atomics is.  Three
-// butterfly steps: lane l
-// takes lane l^1's terms when both stencils start at the same cell (l even),
-// then lane l^2's (bit 1 of l clear), then lane l^4's (bit 2 clear).  Every
-// non-centre TSC term is at most 0.5 * 0.75 * 0.75 of Q(m) < 2^31 (one factor
-// is an outer weight), so four of them sum below 2^32 and the third step adds
-// in 64 bits; the centre remainder, negative for a light enough particle,
-// merges as int64.  Every lane of the wave must be active (the steps read
-// other lanes); ok = the lane holds an entry, corner = its stencil's first
-// cell in the tile.
+// The 27-cell deposit (TSC, or a table-mode narrow entry) with lanes merged
+// first.  Clustered particles put many lanes of a wave on the same stencil,
+// and the LDS serialises adds to one address; the terms can be summed in
+// registers instead.  Deterministic mode: the terms are exact integers, so
+// the result is the same.  Atomic mode: an FP32 sum in another order, as any
+// order of its LDS atomics is.
+// Three butterfly steps: lane l takes lane l^1's terms when both stencils
+// start at the same cell (l even), then lane l^2's (bit 1 of l clear), then
+// lane l^4's (bit 2 clear).  Every non-centre TSC term is at most
+// 0.5 * 0.75 * 0.75 of Q(m) < 2^31 (one factor is an outer weight; table mode
+// merges only when the plan found every non-centre product of its row-0
+// weights at most 0.45), so four of them sum below 2^32 and the third step
+// adds in 64 bits; the centre remainder, negative for a light enough
+// particle, merges as int64.
+// Every lane of the wave must be active (the steps read other lanes); ok =
+// the lane holds an entry, corner = its stencil's first cell in the tile.
 template <int RED, int HX, int HY>
 __device__ __forceinline__ void deposit27_merged(typename AccT<RED>::T* acc, int corner, bool ok, float m,
                                                  const float wx[3], const float wy[3], const float wz[3],
@@ -874,7 +876,7 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
         const int lz = clampi(centre_local(ci[2], k0c, ez, G.n[2], periodic), -1, TZ);
         // corner (-1,-1,-1) of the stencil; halo coordinates are local + 2
         const int corner = (lx + 1) + HX * ((ly + 1) + HY * (lz + 1));
-        if (MODE == 2)
+        if (MODE == 2 || P.merge_narrow)
           deposit27_merged<RED, HX, HY>(acc, corner, ok, m, wx, wy, wz, scale);
         else if (ok)
           deposit27<RED, HX, HY>(acc + corner, m, wx, wy, wz, scale);
