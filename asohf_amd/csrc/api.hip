@@ -488,6 +488,16 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
     BoxRec& r = H.hrec[i];
     for (int a = 0; a < 3; ++a) { r.lo[a] = G.lo[a]; r.hi[a] = G.hi[a]; }
     r.g = H.hlist[i];
+    r.level = G.level;
+    for (int a = 0; a < 3; ++a) {
+      r.org[a] = G.org[a];
+      r.d[a] = G.d[a];
+      r.inv_d[a] = G.inv_d[a];
+      r.n[a] = G.n[a];
+    }
+    r.tile0 = G.tile0;
+    r.nt0 = G.nt[0];
+    r.nt1 = G.nt[1];
     r.pad = 0;
   }
   // ---- upload ----

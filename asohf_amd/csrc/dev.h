@@ -254,13 +254,12 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
       const BoxRec r = P.hrec[se.x + q];
       if (!in_box(r, x, y, z)) continue;
       const int g = r.g;
-      const GridDesc& Gref = P.grids[g];
-      const float gorg[3] = {Gref.org[0], Gref.org[1], Gref.org[2]};
-      const float gd[3] = {Gref.d[0], Gref.d[1], Gref.d[2]};
-      const float gid[3] = {Gref.inv_d[0], Gref.inv_d[1], Gref.inv_d[2]};
-      const int gn[3] = {Gref.n[0], Gref.n[1], Gref.n[2]};
-      const int gtile0 = Gref.tile0, gnt0 = Gref.nt[0], gnt1 = Gref.nt[1];
-      const int glevel = Gref.level;
+      const float gorg[3] = {r.org[0], r.org[1], r.org[2]};
+      const float gd[3] = {r.d[0], r.d[1], r.d[2]};
+      const float gid[3] = {r.inv_d[0], r.inv_d[1], r.inv_d[2]};
+      const int gn[3] = {r.n[0], r.n[1], r.n[2]};
+      const int gtile0 = r.tile0, gnt0 = r.nt0, gnt1 = r.nt1;
+      const int glevel = r.level;
       any = true;
       int lo[3], hi[3], ci[3];
       int irk = 0;

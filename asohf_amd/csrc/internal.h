@@ -40,14 +40,24 @@ struct HashSlot {
   int count;
 };
 
-// A candidate patch of a bucket: its membership box inline, so the box test
-// needs no load of the grid descriptor.
+// A candidate patch of a bucket: its membership box and the grid fields a
+// member particle needs, inline, so neither the box test nor the member's
+// indices and tiles wait on a load of the grid descriptor (96 bytes, six
+// 16-byte loads issued together).
 struct BoxRec {
   float lo[3];
   float hi[3];
   int g;                     // grid id (1 + patch)
+  int level;                 // IR
+  float org[3];              // GridDesc::org, d, inv_d, n, tile0, nt[0], nt[1] of grid g
+  float d[3];
+  float inv_d[3];
+  int n[3];
+  int tile0;
+  int nt0, nt1;
   int pad;
 };
+static_assert(sizeof(BoxRec) == 96, "six 16-byte loads");
 
 // Per-level spatial hash of patch membership boxes.
 struct LevelHash {
