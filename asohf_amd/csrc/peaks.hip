@@ -68,23 +68,72 @@ __global__ void __launch_bounds__(256) peak_scan_base(const float* u, int nx, in
 }
 
 // Patches of one level: one workgroup per patch at a time.
+// AMR levels: a workgroup per patch at a time.  Deep levels hold millions of
+// candidates, and one global counter taking an atomic per wave serialises in
+// L2, so a workgroup collects its candidates in LDS over whole passes of 256
+// cells and reserves space in the output once per kPeakBuf of them (the
+// output order does not matter: the two sorts fix it).
+constexpr int kPeakBuf = 1024;
 __global__ void __launch_bounds__(256) peak_scan_patches(const float* u, const int* conta, const PeakPatch* pp,
                                                          int np, long long ld1, long long ld2, float peak,
                                                          unsigned long long* rec_r, unsigned* rec_k,
                                                          unsigned long long cap, unsigned long long* count) {
+  __shared__ unsigned long long buf_r[kPeakBuf];
+  __shared__ unsigned buf_k[kPeakBuf];
+  __shared__ unsigned s_n;
+  __shared__ unsigned long long s_base;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  // moves the buffer to the output; every thread of the workgroup calls it
+  auto flush = [&]() {
+    const unsigned n = s_n;
+    if (n == 0) return;
+    if (tid == 0) s_base = atomicAdd(count, (unsigned long long)n);
+    __syncthreads();
+    const unsigned long long b = s_base;
+    for (unsigned j = tid; j < n; j += nthr)
+      if (b + j < cap) {
+        rec_r[b + j] = buf_r[j];
+        rec_k[b + j] = buf_k[j];
+      }
+    __syncthreads();
+    if (tid == 0) s_n = 0;
+    __syncthreads();
+  };
   for (int p = blockIdx.x; p < np; p += gridDim.x) {
     const PeakPatch P = pp[p];
     const long long s1 = ld1 > 0 ? ld1 : P.n[0];
     const long long s2 = ld2 > 0 ? ld2 : (long long)P.n[0] * P.n[1];
     const int cells = P.n[0] * P.n[1] * P.n[2];
-    for (int l = threadIdx.x; l < cells; l += blockDim.x) {
-      const int ix = l % P.n[0], jy = (l / P.n[0]) % P.n[1], kz = l / (P.n[0] * P.n[1]);
-      if (ix < 1 || ix > P.n[0] - 2 || jy < 1 || jy > P.n[1] - 2 || kz < 1 || kz > P.n[2] - 2) continue;
-      const long long c = P.off + ix + s1 * jy + s2 * kz;
-      if (conta && conta[c] != 1) continue;
-      if (peak_test(u, c, s1, s2, peak)) peak_append((unsigned long long)(P.rank + l), u[c], rec_r, rec_k, cap, count);
+    for (int l0 = 0; l0 < cells; l0 += nthr) {   // the same passes on every thread
+      const int l = l0 + tid;
+      bool cand = false;
+      float v = 0.f;
+      if (l < cells) {
+        const int ix = l % P.n[0], jy = (l / P.n[0]) % P.n[1], kz = l / (P.n[0] * P.n[1]);
+        if (ix >= 1 && ix <= P.n[0] - 2 && jy >= 1 && jy <= P.n[1] - 2 && kz >= 1 && kz <= P.n[2] - 2) {
+          const long long c = P.off + ix + s1 * jy + s2 * kz;
+          if (!conta || conta[c] == 1) {
+            cand = peak_test(u, c, s1, s2, peak);
+            v = u[c];
+          }
+        }
+      }
+      if (cand) {
+        const unsigned j = atomicAdd(&s_n, 1u);   // below kPeakBuf: flushed before a pass could overflow
+        buf_r[j] = (unsigned long long)(P.rank + l);
+        buf_k[j] = __float_as_uint(1.0f / v);
+      }
+      __syncthreads();
+      // every thread reads the count before any adds to it again, so all take
+      // the same decision
+      const unsigned n_now = s_n;
+      __syncthreads();
+      if (n_now > (unsigned)(kPeakBuf - nthr)) flush();
     }
   }
+  flush();
 }
 
 // Sorted records to the caller's arrays: 1-based cell indices and the
