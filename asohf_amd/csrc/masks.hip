@@ -174,6 +174,8 @@ __device__ __forceinline__ void vg_box(const MaskPatch& P, const float d[3], con
   }
 }
 
+constexpr int kVgTile = 256;   // threads of vg_neighbours, and candidates per LDS tile
+
 // Neighbours of each level patch (J != I, lattice boxes intersecting), in
 // ascending J.  Pass 0 counts them; pass 1 writes them with the pair's
 // overlap from the reference's CORN expressions (grids.f:2372-2453): per
@@ -181,21 +183,42 @@ __device__ __forceinline__ void vg_box(const MaskPatch& P, const float d[3], con
 __global__ void __launch_bounds__(256) vg_neighbours(const MaskPatch* mp, int lo, int np, VgGeo G, int pass,
                                                      unsigned* count, const unsigned long long* start,
                                                      VgPair* out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= np) return;
-  const MaskPatch P = mp[lo + k];
-  int gi[6], gj[6];
-  float r1[3], r2[3], q1[3], q2[3];
-  vg_box(P, G.d, G.fix, gi, r1, r2);
+  // candidates J come in tiles of kVgTile: each one's lattice box is computed
+  // once per workgroup into LDS and tested by every thread; a pair that
+  // overlaps recomputes J's box in full for the overlap expressions.  The
+  // tiles and the candidates within them run in ascending J.
+  __shared__ int sbox[6][kVgTile];
+  const int k = blockIdx.x * kVgTile + threadIdx.x;
+  const bool active = k < np;
+  int gi[6] = {0, 0, 0, 0, 0, 0}, gj[6];
+  float r1[3] = {0.f, 0.f, 0.f}, r2[3] = {0.f, 0.f, 0.f}, q1[3], q2[3];
+  if (active) vg_box(mp[lo + k], G.d, G.fix, gi, r1, r2);
   unsigned c = 0;
-  unsigned long long w = pass ? start[k] : 0;
-  for (int j = 0; j < np; ++j) {
-    if (j == k) continue;
-    const MaskPatch Q = mp[lo + j];
-    vg_box(Q, G.d, G.fix, gj, q1, q2);
-    if (!(gi[0] <= gj[3] && gj[0] <= gi[3] && gi[1] <= gj[4] && gj[1] <= gi[4] && gi[2] <= gj[5] && gj[2] <= gi[5]))
-      continue;
-    if (pass) {
+  const unsigned long long w = pass && active ? start[k] : 0;
+  for (int j0 = 0; j0 < np; j0 += kVgTile) {
+    __syncthreads();   // every thread is done with the previous tile
+    if (j0 + (int)threadIdx.x < np) {
+      int gt[6];
+      float t1[3], t2[3];
+      vg_box(mp[lo + j0 + threadIdx.x], G.d, G.fix, gt, t1, t2);
+#pragma unroll
+      for (int a = 0; a < 6; ++a) sbox[a][threadIdx.x] = gt[a];
+    }
+    __syncthreads();
+    if (!active) continue;
+    const int jn = min(kVgTile, np - j0);
+    for (int t = 0; t < jn; ++t) {
+      const int j = j0 + t;
+      if (j == k) continue;
+      if (!(gi[0] <= sbox[3][t] && sbox[0][t] <= gi[3] && gi[1] <= sbox[4][t] && sbox[1][t] <= gi[4] &&
+            gi[2] <= sbox[5][t] && sbox[2][t] <= gi[5]))
+        continue;
+      if (!pass) {
+        ++c;
+        continue;
+      }
+      const MaskPatch Q = mp[lo + j];
+      vg_box(Q, G.d, G.fix, gj, q1, q2);
       VgPair v;
       v.j = j;
 #pragma unroll
@@ -229,10 +252,10 @@ __global__ void __launch_bounds__(256) vg_neighbours(const MaskPatch* mp, int lo
         v.len[a] = cc2 - cc1;
       }
       out[w + c] = v;
+      ++c;
     }
-    ++c;
   }
-  if (!pass) count[k] = c;
+  if (active && !pass) count[k] = c;
 }
 
 constexpr int kVgCover = 16;   // patches of one level covering one cell
@@ -329,7 +352,7 @@ hipError_t launch_cr0amr(const MaskPatch* mp, int np, int nlev1, const int* lev_
 hipError_t launch_vg_neighbours(const MaskPatch* mp, int lo, int np, const VgGeo& G, int pass, unsigned* count,
                                 const unsigned long long* start, VgPair* out, hipStream_t s) {
   if (np <= 0) return hipSuccess;
-  vg_neighbours<<<(np + 255) / 256, 256, 0, s>>>(mp, lo, np, G, pass, count, start, out);
+  vg_neighbours<<<(np + kVgTile - 1) / kVgTile, kVgTile, 0, s>>>(mp, lo, np, G, pass, count, start, out);
   return hipGetLastError();
 }
 
