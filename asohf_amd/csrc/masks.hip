@@ -93,66 +93,87 @@ __global__ void __launch_bounds__(256) cr_mothers(const MaskPatch* mp, int np, i
 // two cells), then mark, per listed patch, the cells in a window around the
 // box whose centres pass the reference's exact test.
 constexpr int kCrList = 512;
+constexpr int kCrGroup = 4;   // fine patches tested together against each coarse patch
 __global__ void __launch_bounds__(256) cr_cross(const MaskPatch* mp, int lo, int n_ir, int lo2, int n_ir1,
                                                 float dxpa, float dypa, float dzpa, int* cr11, long long s1p,
                                                 long long s2p, int* err) {
-  __shared__ int s_list[kCrList];
-  __shared__ int s_n;
+  __shared__ int s_list[kCrGroup][kCrList];
+  __shared__ int s_n[kCrGroup];
   const float d[3] = {dxpa, dypa, dzpa};
-  for (int w = blockIdx.x; w < n_ir1; w += gridDim.x) {
-    const MaskPatch F = mp[lo2 + w];
-    float b1[3], b2[3];
+  const int ngroups = (n_ir1 + kCrGroup - 1) / kCrGroup;
+  // a workgroup takes kCrGroup consecutive fine patches at a time, so each
+  // coarse patch is loaded and its box formed once for all of them
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int wg = grp * kCrGroup;
+    const int gn = min(kCrGroup, n_ir1 - wg);
+    float b1[kCrGroup][3], b2[kCrGroup][3];
+    int par[kCrGroup];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      // XX1 = PATCHRX(II) - 0.5*(DXPA*0.5); XX2 = XX1 + (PATCHNX(II)-1)*DXPA*0.5
-      b1[a] = F.pr[a] - 0.5f * (d[a] * 0.5f);
-      b2[a] = b1[a] + ((float)(F.n[a] - 1) * d[a]) * 0.5f;
+    for (int g = 0; g < kCrGroup; ++g) {
+      const MaskPatch F = mp[lo2 + wg + min(g, gn - 1)];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        // XX1 = PATCHRX(II) - 0.5*(DXPA*0.5); XX2 = XX1 + (PATCHNX(II)-1)*DXPA*0.5
+        b1[g][a] = F.pr[a] - 0.5f * (d[a] * 0.5f);
+        b2[g][a] = b1[g][a] + ((float)(F.n[a] - 1) * d[a]) * 0.5f;
+      }
+      par[g] = F.pare;
     }
-    if (threadIdx.x == 0) s_n = 0;
+    if (threadIdx.x < kCrGroup) s_n[threadIdx.x] = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < n_ir; i += blockDim.x) {
       const MaskPatch P = mp[lo + i];
-      if (P.self == F.pare) continue;   // children were marked as mothers
-      bool hit = true;
+      float c0[3], c1[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float c0 = P.pr[a] - 0.5f * d[a];
-        const float c1 = c0 + (float)(P.n[a] - 1) * d[a];
-        hit = hit && (c1 >= b1[a] - 2.0f * d[a]) && (c0 <= b2[a] + 2.0f * d[a]);
+        c0[a] = P.pr[a] - 0.5f * d[a];
+        c1[a] = c0[a] + (float)(P.n[a] - 1) * d[a];
       }
-      if (hit) {
-        const int k = atomicAdd(&s_n, 1);
-        if (k < kCrList) s_list[k] = lo + i;
-        else atomicOr(err, 4);
+#pragma unroll
+      for (int g = 0; g < kCrGroup; ++g) {
+        if (g >= gn || P.self == par[g]) continue;   // children were marked as mothers
+        bool hit = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          hit = hit && (c1[a] >= b1[g][a] - 2.0f * d[a]) && (c0[a] <= b2[g][a] + 2.0f * d[a]);
+        if (hit) {
+          const int k = atomicAdd(&s_n[g], 1);
+          if (k < kCrList) s_list[g][k] = lo + i;
+          else atomicOr(err, 4);
+        }
       }
     }
     __syncthreads();
-    const int nl = min(s_n, kCrList);
-    for (int k = 0; k < nl; ++k) {
-      const MaskPatch P = mp[s_list[k]];
-      int w0[3], wn[3];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const float c0 = P.pr[a] - 0.5f * d[a];
-        int i0 = (int)floorf((b1[a] - c0) / d[a]) - 1, i1 = (int)ceilf((b2[a] - c0) / d[a]) + 3;
-        i0 = max(i0 + 1, 1);
-        i1 = min(i1, P.n[a]);
-        w0[a] = i0;
-        wn[a] = max(i1 - i0 + 1, 0);
-      }
-      const long long t1 = s1p > 0 ? s1p : P.n[0], t2 = s2p > 0 ? s2p : (long long)P.n[0] * P.n[1];
-      const int cells = wn[0] * wn[1] * wn[2];
-      for (int l = threadIdx.x; l < cells; l += blockDim.x) {
-        const int ix = w0[0] + l % wn[0], jy = w0[1] + (l / wn[0]) % wn[1], kz = w0[2] + l / (wn[0] * wn[1]);
-        const int idx[3] = {ix, jy, kz};
-        bool in = true;
+    for (int g = 0; g < kCrGroup; ++g) {   // unrolled: the boxes stay in registers
+      if (g >= gn) break;
+      const int nl = min(s_n[g], kCrList);
+      for (int k = 0; k < nl; ++k) {
+        const MaskPatch P = mp[s_list[g][k]];
+        int w0[3], wn[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          // XX = PATCHRX(I) - 0.5*DXPA + (IX-1)*DXPA; (XX-XX1)*(XX2-XX) >= 0
-          const float xx = (P.pr[a] - 0.5f * d[a]) + (float)(idx[a] - 1) * d[a];
-          in = in && ((xx - b1[a]) * (b2[a] - xx) >= 0.0f);
+          const float c0 = P.pr[a] - 0.5f * d[a];
+          int i0 = (int)floorf((b1[g][a] - c0) / d[a]) - 1, i1 = (int)ceilf((b2[g][a] - c0) / d[a]) + 3;
+          i0 = max(i0 + 1, 1);
+          i1 = min(i1, P.n[a]);
+          w0[a] = i0;
+          wn[a] = max(i1 - i0 + 1, 0);
         }
-        if (in) cr11[P.off + (ix - 1) + t1 * (jy - 1) + t2 * (kz - 1)] = 0;
+        const long long t1 = s1p > 0 ? s1p : P.n[0], t2 = s2p > 0 ? s2p : (long long)P.n[0] * P.n[1];
+        const int cells = wn[0] * wn[1] * wn[2];
+        for (int l = threadIdx.x; l < cells; l += blockDim.x) {
+          const int ix = w0[0] + l % wn[0], jy = w0[1] + (l / wn[0]) % wn[1], kz = w0[2] + l / (wn[0] * wn[1]);
+          const int idx[3] = {ix, jy, kz};
+          bool in = true;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            // XX = PATCHRX(I) - 0.5*DXPA + (IX-1)*DXPA; (XX-XX1)*(XX2-XX) >= 0
+            const float xx = (P.pr[a] - 0.5f * d[a]) + (float)(idx[a] - 1) * d[a];
+            in = in && ((xx - b1[g][a]) * (b2[g][a] - xx) >= 0.0f);
+          }
+          if (in) cr11[P.off + (ix - 1) + t1 * (jy - 1) + t2 * (kz - 1)] = 0;
+        }
       }
     }
     __syncthreads();
@@ -343,7 +364,8 @@ hipError_t launch_cr0amr(const MaskPatch* mp, int np, int nlev1, const int* lev_
     for (int k = 0; k < ir; ++k) p2 *= 2.0f;
     const int n1 = lev_n[ir + 1];
     if (lev_n[ir] == 0 || n1 == 0) continue;
-    cr_cross<<<n1 < cus * 16 ? n1 : cus * 16, 256, 0, s>>>(mp, lev_lo[ir], lev_n[ir], lev_lo[ir + 1], n1, dx / p2,
+    const int ng = (n1 + kCrGroup - 1) / kCrGroup;
+    cr_cross<<<ng < cus * 16 ? ng : cus * 16, 256, 0, s>>>(mp, lev_lo[ir], lev_n[ir], lev_lo[ir + 1], n1, dx / p2,
                                                            dy / p2, dz / p2, cr11, s1p, s2p, err);
   }
   return hipGetLastError();
