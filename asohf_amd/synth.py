@@ -66,7 +66,7 @@ class SnapshotSpec:
 # The workloads of the density build: 1 is the small CPU-runnable case, 2 the
 # 256^3-particle single-GPU benchmark, 3 and 5 the large multi-level runs, 4 the
 # base grid alone (no refinement).
-CONFIGS = {
+WORKLOADS = {
     1: SnapshotSpec("c1_128cubed_3lev", 128, 40.0, 128, (24, 18, 12), 200, 0.70, 1e11, 1e14, 1),
     2: SnapshotSpec("c2_256cubed_4lev", 256, 40.0, 256, (320, 260, 200, 140), 200, 0.70,
                     1e11, 1e14, 2),

@@ -31,7 +31,7 @@ Prints ONE JSON line on stdout; progress goes to stderr.
              (INTERPOLATE_DENSITY_TSC, FP32) on this host's cores; whole for
              the 2M/16.8M-particle workloads, extrapolated from two particle
              samples for the 134M/1.07G ones
-Workloads (--config): asohf_amd.synth.CONFIGS; 2 (256^3 particles, 256^3 base
+Workloads (--config): asohf_amd.synth.WORKLOADS; 2 (256^3 particles, 256^3 base
 grid, 4 AMR levels) is the default.
 """
 from __future__ import annotations
@@ -58,7 +58,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=2, help="workload id (asohf_amd.synth.CONFIGS)")
+    ap.add_argument("--config", type=int, default=2, help="workload id (asohf_amd.synth.WORKLOADS)")
     ap.add_argument("--mode", default="tsc", choices=["tsc", "table"])
     ap.add_argument("--reduction", default="deterministic", choices=["deterministic", "atomic"])
     ap.add_argument("--plain-steps", type=int, default=3)
@@ -109,7 +109,7 @@ def main() -> int:
     from asohf_amd import synth
     from asohf_amd.density import DensityBuilder
 
-    spec = synth.CONFIGS[args.config]
+    spec = synth.WORKLOADS[args.config]
     dev = torch.device("cuda:0")
     t0 = time.time()
     if spec.n > (1 << 25):

@@ -134,10 +134,10 @@ def main() -> None:
     if not O.ref_available():
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
     for seed in (11, 12):
-        snap = synth.make_snapshot(synth.CONFIGS[1], n_side=20, nx=16, patches_per_level=(3, 3, 2),
+        snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=20, nx=16, patches_per_level=(3, 3, 2),
                                    side_hi=32, seed=seed)
         save(f"seeded_{seed}", snap)
-    snap = synth.make_snapshot(synth.CONFIGS[1], n_side=32, nx=32, patches_per_level=(5, 4, 3),
+    snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=32, nx=32, patches_per_level=(5, 4, 3),
                                side_hi=32, seed=13, interp_degree=1)
     save("seeded_13_cic", snap)
     save("edges", edges_case())

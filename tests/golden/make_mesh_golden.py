@@ -29,7 +29,7 @@ CASES = [
 
 
 def snapshot(c):
-    return synth.make_snapshot(synth.CONFIGS[1], n_side=c["n_side"], nx=c["nx"], patches_per_level=(3, 2, 1),
+    return synth.make_snapshot(synth.WORKLOADS[1], n_side=c["n_side"], nx=c["nx"], patches_per_level=(3, 2, 1),
                                seed=c["seed"])
 
 

@@ -65,7 +65,7 @@ def test_mesh_params_layout_matches_header():
 
 
 def test_synthetic_hierarchy_obeys_mesh_rules():
-    snap = synth.make_snapshot(synth.CONFIGS[2], n_side=40, nx=64, patches_per_level=(20, 15, 10, 8), seed=9)
+    snap = synth.make_snapshot(synth.WORKLOADS[2], n_side=40, nx=64, patches_per_level=(20, 15, 10, 8), seed=9)
     g = snap.geom
     assert np.all(g.pnx % 2 == 0) and np.all(g.pny % 2 == 0) and np.all(g.pnz % 2 == 0)
     lev = g.level_of_patch()
@@ -89,7 +89,7 @@ def test_synthetic_hierarchy_obeys_mesh_rules():
 
 
 def test_species_partition_contiguous():
-    snap = synth.make_snapshot(synth.CONFIGS[1], n_side=32, nx=32, patches_per_level=(4, 3, 2), seed=3)
+    snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=32, nx=32, patches_per_level=(4, 3, 2), seed=3)
     assert snap.npart_esp.sum() == snap.n
     assert snap.npart_esp[0] > 0 and snap.npart_esp[1:].sum() > 0
 

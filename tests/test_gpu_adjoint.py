@@ -51,7 +51,7 @@ def test_adjoint_vs_fp64_oracle(bld, name):
 
 
 def test_adjoint_vs_central_differences_large(bld):
-    spec = synth.CONFIGS[4]
+    spec = synth.WORKLOADS[4]
     snap = synth.make_snapshot(spec)
     g = snap.geom
     rng = np.random.default_rng(spec.seed)
@@ -114,7 +114,7 @@ def _dense_torch_deposit(x, y, z, m, geom):
 
 def test_autograd_vs_dense_torch(bld):
     from asohf_amd.density import tsc_deposit
-    snap = synth.make_snapshot(synth.CONFIGS[4], n_side=32, nx=32)
+    snap = synth.make_snapshot(synth.WORKLOADS[4], n_side=32, nx=32)
     g = snap.geom
     rng = np.random.default_rng(4)
     w = rng.standard_normal((g.nz, g.ny, g.nx))

@@ -25,7 +25,7 @@ MODES = {"table": "A", "tsc": "B"}
 
 
 def small(seed=11, n_side=32):
-    return synth.make_snapshot(synth.CONFIGS[1], n_side=n_side, nx=32, patches_per_level=(4, 3, 2), seed=seed)
+    return synth.make_snapshot(synth.WORKLOADS[1], n_side=n_side, nx=32, patches_per_level=(4, 3, 2), seed=seed)
 
 
 def to_dev(snap, k=None):

@@ -127,7 +127,7 @@ def test_clustered_stencils_merge_exactly(bld):
     binary exponent of the largest mass), so the base-grid mass equals
     sum(Q) / scale exactly; the fields are bit-identical under a shuffled
     particle order and across tile shapes, and they match the FP64 oracle."""
-    src = synth.make_snapshot(synth.CONFIGS[1], n_side=16, nx=32, patches_per_level=(3, 2), seed=5)
+    src = synth.make_snapshot(synth.WORKLOADS[1], n_side=16, nx=32, patches_per_level=(3, 2), seed=5)
     g = src.geom
     rng = np.random.default_rng(11)
     dx = float(g.dx)
@@ -231,7 +231,7 @@ def test_edge_cases_empty_and_no_levels(bld):
 
 
 def test_larger_snapshot_properties(bld):
-    snap = synth.make_snapshot(synth.CONFIGS[1], n_side=64, nx=64, patches_per_level=(30, 25, 20), seed=4)
+    snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=64, nx=64, patches_per_level=(30, 25, 20), seed=4)
     for mode in ("table", "tsc"):
         print(f"case larger {mode}", flush=True)
         u1, u11, gm = run(bld, snap, mode)
@@ -249,7 +249,7 @@ def test_full_size_workload_properties(bld):
     properties (exact mass conservation, bit-identical results under a
     reversed particle order, across runs and in the graph form) and FP64
     oracle parity on a seeded 2^17-particle sample with the full hierarchy."""
-    snap = synth.make_snapshot(synth.CONFIGS[2])
+    snap = synth.make_snapshot(synth.WORKLOADS[2])
     g = snap.geom
     print(f"case full-size c2: {snap.n} particles, {g.npatch_total} patches", flush=True)
     d = torch.device("cuda:0")
@@ -300,7 +300,7 @@ def test_overlapping_patches_overflow_emission_lists(bld, mode):
     holds, so the scatter pass enumerates those workgroups again; the fields
     still match the FP64 oracle and every copy gets the same density (both
     modes keep emission lists)."""
-    snap = synth.make_snapshot(synth.CONFIGS[1], n_side=32, nx=32, patches_per_level=(4, 3, 2), seed=21)
+    snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=32, nx=32, patches_per_level=(4, 3, 2), seed=21)
     g = snap.geom
     k = int(g.npatch[1])
     rep = 12

@@ -29,7 +29,7 @@ f32 = np.float32
 def cases():
     for seed in range(4):
         yield f"overlaps_{seed}", random_hierarchy(np.random.default_rng(seed))
-    yield "synth_c1", synth.make_snapshot(synth.CONFIGS[1], n_side=32, nx=64, patches_per_level=(8, 6, 4), seed=8,
+    yield "synth_c1", synth.make_snapshot(synth.WORKLOADS[1], n_side=32, nx=64, patches_per_level=(8, 6, 4), seed=8,
                                           side_hi=32)
 
 
@@ -100,7 +100,7 @@ def _interiors(g, padded):
 def test_gpu_masks_match_oracle():
     from asohf_amd.density import DensityBuilder
     b = DensityBuilder(0)
-    all_cases = list(cases()) + [("c2_full", synth.make_snapshot(synth.CONFIGS[2]))]
+    all_cases = list(cases()) + [("c2_full", synth.make_snapshot(synth.WORKLOADS[2]))]
     for name, snap in all_cases:
         g = snap.geom
         mc0, mc11 = O.cr0amr(g)

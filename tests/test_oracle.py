@@ -38,7 +38,7 @@ def test_fp32_bitexact_vs_reference_fixtures(golden, mode):
 @pytest.mark.parametrize("seed", [21, 22])
 @pytest.mark.parametrize("mode", ["A", "B"])
 def test_fp32_bitexact_vs_reference_fresh(seed, mode):
-    snap = synth.make_snapshot(synth.CONFIGS[1], n_side=24, nx=24, patches_per_level=(4, 3, 3),
+    snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=24, nx=24, patches_per_level=(4, 3, 3),
                                side_hi=32, seed=seed)
     _, _, _, radx, r1, r11 = O.run_ref(snap, mode)
     u1, u11 = O.density32(snap, mode)

@@ -162,7 +162,7 @@ def test_gpu_peaks_synthetic_fields_both_layouts():
     from asohf_amd.density import DensityBuilder
     rng = np.random.default_rng(7)
     b = DensityBuilder(0)
-    snap = synth.make_snapshot(synth.CONFIGS[1], n_side=24, nx=32, patches_per_level=(5, 4, 3), seed=3)
+    snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=24, nx=32, patches_per_level=(5, 4, 3), seed=3)
     g = snap.geom
     # base grid: quantized values (ties, plateaus), packed and with padded leading dims
     u1 = quantized_field(rng, (g.nz, g.ny, g.nx))
@@ -218,9 +218,9 @@ def test_gpu_peaks_on_density_fields(which):
     from asohf_amd.density import DensityBuilder
     b = DensityBuilder(0)
     if which == "golden":
-        snap = synth.make_snapshot(synth.CONFIGS[1], n_side=48, nx=64, patches_per_level=(6, 4, 2), seed=11)
+        snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=48, nx=64, patches_per_level=(6, 4, 2), seed=11)
     else:
-        snap = synth.make_snapshot(synth.CONFIGS[2])
+        snap = synth.make_snapshot(synth.WORKLOADS[2])
     g = snap.geom
     x, y, z, m = (_dev(a) for a in (snap.x, snap.y, snap.z, snap.m))
     r = b.density(g, x, y, z, m, snap.npart_esp, mode="tsc")

@@ -22,8 +22,8 @@ def consts():
 
 
 def snapshots():
-    yield "seeded_c1", synth.make_snapshot(synth.CONFIGS[1], n_side=48, nx=64, patches_per_level=(3, 2, 1), seed=5)
-    yield "seeded_c1_b", synth.make_snapshot(synth.CONFIGS[1], n_side=40, nx=128, patches_per_level=(2, 1), seed=9)
+    yield "seeded_c1", synth.make_snapshot(synth.WORKLOADS[1], n_side=48, nx=64, patches_per_level=(3, 2, 1), seed=5)
+    yield "seeded_c1_b", synth.make_snapshot(synth.WORKLOADS[1], n_side=40, nx=128, patches_per_level=(2, 1), seed=9)
 
 
 def edge_particles():
@@ -171,7 +171,7 @@ def test_gpu_species_matches_oracle():
     cases.append(("edges",) + edge_particles())
     cases.append(("threshold_cells",) + threshold_cells(True))
     cases.append(("threshold_cells_reversed",) + threshold_cells(False))
-    big = synth.make_snapshot(synth.CONFIGS[2])
+    big = synth.make_snapshot(synth.WORKLOADS[2])
     cases.append(("c2_full", big.x, big.y, big.z, big.m, float(big.geom.dx)))
     for name, x, y, z, m, dx in cases:
         print(f"case species {name} n={len(x)}", flush=True)
