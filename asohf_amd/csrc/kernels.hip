@@ -632,7 +632,8 @@ __device__ __forceinline__ unsigned from_xor1(unsigned v) {
 __device__ __forceinline__ unsigned from_xor2(unsigned v) {
   return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
 }
-// the value of lane l+4 (l-4) of the same row of 16 lanes
+// the value of lane l+4 (l-4) of the same row of 16 lanes; a lane whose
+// source lies outside its row gets 0 (the DPP default)
 __device__ __forceinline__ unsigned from_plus4(unsigned v) {
   return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x104, 0xF, 0xF, false);   // row_shl:4
 }
