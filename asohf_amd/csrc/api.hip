@@ -498,7 +498,12 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
     r.tile0 = G.tile0;
     r.nt0 = G.nt[0];
     r.nt1 = G.nt[1];
+    r.den = G.den;
+    for (int a = 0; a < 3; ++a) r.pr[a] = G.pr[a];
+    r.ld1 = G.ld1;
+    r.ld2 = G.ld2;
     r.pad = 0;
+    r.off = G.off;
   }
   // ---- upload ----
   size_t off = 0;

@@ -42,22 +42,27 @@ struct HashSlot {
 
 // A candidate patch of a bucket: its membership box and the grid fields a
 // member particle needs, inline, so neither the box test nor the member's
-// indices and tiles wait on a load of the grid descriptor (96 bytes, six
-// 16-byte loads issued together).
+// indices, tiles and output cells wait on a load of the grid descriptor
+// (128 bytes, eight 16-byte loads issued together).
 struct BoxRec {
   float lo[3];
   float hi[3];
   int g;                     // grid id (1 + patch)
   int level;                 // IR
-  float org[3];              // GridDesc::org, d, inv_d, n, tile0, nt[0], nt[1] of grid g
-  float d[3];
+  float org[3];              // GridDesc::org, d, inv_d, n, tile0, nt[0], nt[1], pr, den, ld1, ld2,
+  float d[3];                // off of grid g
   float inv_d[3];
   int n[3];
   int tile0;
   int nt0, nt1;
+  float den;
+  float pr[3];
+  int ld1;
+  int ld2;
   int pad;
+  long long off;
 };
-static_assert(sizeof(BoxRec) == 96, "six 16-byte loads");
+static_assert(sizeof(BoxRec) == 128, "eight 16-byte loads");
 
 // Per-level spatial hash of patch membership boxes.
 struct LevelHash {

@@ -1180,7 +1180,7 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
         if (!in_box(r, p[0], p[1], p[2])) continue;
         any = true;
         const int g = r.g;
-        const GridDesc& G = P.grids[g];
+        const BoxRec& G = r;   // the grid's fields travel in its candidate record
         int ci[3];
         float w[3][3], dw[3][3];
 #pragma unroll
