@@ -625,7 +625,10 @@ static asohf_status decode_err(asohf_ctx* ctx, unsigned e) {
   std::string m = "invalid particle data:";
   if (e & kErrNonFinite) m += " non-finite position or mass;";
   if (e & kErrNegMass) m += " negative mass;";
-  if (e & kErrWrap) m += " base-grid cell index outside the single periodic wrap range of DENSITY;";
+  if (e & kErrWrap)
+    m += " base-grid position outside the range the mode accepts (kernel-table mode: cell index outside "
+         "the single periodic wrap of DENSITY; TSC mode: more than half a cell past the first or last "
+         "cell centre);";
   return fail(ctx, ASOHF_ERR_INVALID, m);
 }
 
@@ -1102,6 +1105,9 @@ asohf_status asohf_hip_density_adjoint(asohf_ctx* ctx, const asohf_geom* geom, i
   if (st != ASOHF_OK) return st;
   if (n < 0 || (n > 0 && (!rx || !ry || !rz || !m || !dm || !dxp || !dyp || !dzp)) || !g1)
     return fail(ctx, ASOHF_ERR_INVALID, "NULL array");
+  long long npatch = 0;
+  for (int l = 0; l <= geom->nl; ++l) npatch += geom->npatch[l];
+  if (npatch > 0 && !g11) return fail(ctx, ASOHF_ERR_INVALID, "g11 is NULL but there are patches");
   HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
   st = build_plan(ctx, geom, o.mode, &o);
   if (st != ASOHF_OK) return st;
@@ -1109,7 +1115,13 @@ asohf_status asohf_hip_density_adjoint(asohf_ctx* ctx, const asohf_geom* geom, i
   st = make_parts(ctx, o, n, rx, ry, rz, m, nullptr, A);
   if (st != ASOHF_OK) return st;
   hipStream_t s = (hipStream_t)o.stream;   // NULL: the default stream
+  HIPCHK(ctx->counters.ensure(sizeof(Counters)), "allocating counters");
+  Counters* C = ctx->counters.as<Counters>();
   HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
+  // the forward's input checks (non-finite values, negative masses,
+  // positions outside the TSC base range), read back after the gather
+  HIPCHK(launch_fill_u32((unsigned*)C, 0u, sizeof(Counters) / 4, s), "clearing counters");
+  if (n > 0) HIPCHK(launch_plain_validate(ctx->plan.dev, A, C, ctx->cus, s), "validation");
   // Large inputs are gathered in base-cell order (a radix sort of (cell,
   // index) pairs): neighbouring lanes then read the same gradient rows.
   const unsigned* order = nullptr;
@@ -1135,6 +1147,10 @@ asohf_status asohf_hip_density_adjoint(asohf_ctx* ctx, const asohf_geom* geom, i
   }
   if (n > 0) HIPCHK(launch_adjoint(ctx->plan.dev, A, g1, g11, dm, dxp, dyp, dzp, order, ctx->cus, s), "adjoint kernel");
   HIPCHK(hipEventRecord(ctx->ev[6], s), "event");
+  HIPCHK(hipMemcpyAsync(ctx->hcount, C, sizeof(Counters), hipMemcpyDeviceToHost, s), "reading counters");
+  HIPCHK(hipStreamSynchronize(s), "adjoint");
+  st = decode_err(ctx, ctx->hcount->err);
+  if (st != ASOHF_OK) return st;
   ctx->stats = asohf_stats{};
   ctx->stats.n_particles = n;
   ctx->stats_pending = true;
@@ -1164,9 +1180,11 @@ asohf_status asohf_hip_membership(asohf_ctx* ctx, const asohf_geom* geom, int64_
   hipStream_t s = ctx->stream;
   HIPCHK(launch_fill_u32((unsigned*)c, 0u, 2, s), "clearing");
   HIPCHK(launch_membership(ctx->plan.dev, A, out, cap, c, ctx->cus, s), "membership kernel");
-  HIPCHK(hipMemcpyAsync(ctx->hscalar, c, sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "D2H");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar, c, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "D2H");
   HIPCHK(hipStreamSynchronize(s), "membership");
-  *count = (int64_t)*ctx->hscalar;
+  st = decode_err(ctx, (unsigned)ctx->hscalar[1]);
+  if (st != ASOHF_OK) return st;
+  *count = (int64_t)ctx->hscalar[0];
   return ASOHF_OK;
 }
 

@@ -72,6 +72,24 @@ __device__ __forceinline__ int tsc_base_index(float xp, float xl, float d, float
   return clampi((int)t + 1, 1, n);
 }
 
+// Positions the TSC base deposit accepts.  The reference clamps the base
+// index to [1,N] (grids.f:1184-1192) and evaluates the weights at that cell's
+// centre (grids.f:1194-1213); more than half a cell past the first or last
+// centre the middle weight 0.75 - b^2 shrinks and then turns negative, and
+// the outer ones grow past 1/2, which the exact fixed-point deposit (terms
+// rounded to unsigned 32-bit integers, four of them summed in 32 bits) does
+// not represent.  Such positions lie outside the box [-LADO0/2, LADO0/2] by
+// more than rounding; they are rejected (kErrWrap), as the table mode
+// rejects positions outside its wrap range.  The slack of 2^-10 cells keeps
+// every position of the box (whose quotient may round a few ulps past 0 or
+// N) and bounds |offset| by 1/2 + 2^-10, where every non-centre weight
+// product stays below 0.2822.
+constexpr float kTscBaseSlack = 9.765625e-4f;
+__device__ __forceinline__ bool tsc_base_in_range(float xp, float xl, float inv_d, int n) {
+  const float t = (xp - xl) * inv_d;
+  return t >= -kTscBaseSlack && t <= (float)n + kTscBaseSlack;
+}
+
 __host__ __device__ __forceinline__ uint64_t bucket_key(int level, int bx, int by, int bz) {
   return ((uint64_t)level << 60) | ((uint64_t)bx << 40) | ((uint64_t)by << 20) | (uint64_t)bz;
 }
@@ -210,6 +228,11 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
   // ---------------- base grid ----------------
   int ix, jy, kz;
   if (MODE == 2) {
+    if (!(tsc_base_in_range(x, P.xl, bid[0], bn[0]) && tsc_base_in_range(y, P.xl, bid[1], bn[1]) &&
+          tsc_base_in_range(z, P.xl, bid[2], bn[2]))) {
+      err |= kErrWrap;
+      return;
+    }
     ix = tsc_base_index(x, P.xl, bd[0], bid[0], bn[0]);
     jy = tsc_base_index(y, P.xl, bd[1], bid[1], bn[1]);
     kz = tsc_base_index(z, P.xl, bd[2], bid[2], bn[2]);

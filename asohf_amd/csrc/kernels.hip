@@ -389,6 +389,9 @@ __global__ void __launch_bounds__(256) membership_kernel(Plan P, Parts A, int* o
         },
         err);
   }
+  // validation flags of the enumeration in count[1]
+  err = wave_or(err);
+  if ((threadIdx.x & 63) == 0 && err) atomicOr(count + 1, (unsigned long long)err);
 }
 
 // ---------------------------------------------------------------------------
@@ -1302,6 +1305,11 @@ __global__ void __launch_bounds__(256) plain_validate(Plan P, Parts A, Counters*
     const float m = A.m[i];
     if (!(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]) && isfinite(m))) { err |= kErrNonFinite; continue; }
     if (m < 0.f) err |= kErrNegMass;
+    if (MODE == 2) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        if (!tsc_base_in_range(p[a], P.xl, B.inv_d[a], B.n[a])) err |= kErrWrap;
+    }
     if (MODE == 1) {
 #pragma unroll
       for (int a = 0; a < 3; ++a) {

@@ -312,11 +312,13 @@ class DensityBuilder:
 
     # ------------------------------------------------------------------ adjoint
     def adjoint(self, geom: Geometry, x, y, z, m, g1, g11, *, u1_ld=None, u11_ld=None):
-        """dL/dm, dL/dx, dL/dy, dL/dz of L = sum(g1*u1) + sum(g11*u11), TSC mode."""
+        """dL/dm, dL/dx, dL/dy, dL/dz of L = sum(g1*u1) + sum(g11*u11), TSC mode.
+        g11=None stands for a zero gradient on every patch cell."""
         n = _check_particles(x, y, z, m)
         g1 = g1.contiguous().float()
-        g11 = (g11.contiguous().float() if g11 is not None and g11.numel()
-               else torch.zeros(1, dtype=torch.float32, device=x.device))
+        if g11 is None:
+            g11 = torch.zeros(self.out_shapes(geom, u1_ld, u11_ld)[1], dtype=torch.float32, device=x.device)
+        g11 = g11.contiguous().float()
         outs = [torch.empty(n, dtype=torch.float32, device=x.device) for _ in range(4)]
         ga = _GeomArgs(geom)
         o = self._opts("tsc", "deterministic", "batched", u1_ld, u11_ld)

@@ -50,6 +50,31 @@ def test_adjoint_vs_fp64_oracle(bld, name):
         assert np.max(np.abs(a - b)) <= 1e-4 * scale, nm
 
 
+def test_adjoint_without_patch_gradient(bld):
+    """g11=None is a zero gradient on every patch cell, on a fixture whose
+    particles sit in patches (the gather must not read past a short buffer)."""
+    snap, _ = load_fixture("seeded_12")
+    g = snap.geom
+    assert g.npatch_total > 0
+    rng = np.random.default_rng(2)
+    g1 = rng.standard_normal((g.nz, g.ny, g.nx)).astype(np.float32)
+    x, y, z, m = dev(snap.x, snap.y, snap.z, snap.m)
+    t1, = dev(g1)
+    got = [t.cpu().numpy().astype(np.float64) for t in bld.adjoint(g, x, y, z, m, t1, None)]
+    want = O.adjoint64(snap, g1.astype(np.float64), np.zeros(int(g.patch_cells().sum())))
+    for a, b, nm in zip(got, want, ("dm", "dx", "dy", "dz")):
+        assert np.max(np.abs(a - b)) <= 1e-4 * np.abs(b).max(), nm
+    # the C entry point itself refuses a NULL patch gradient when there are patches
+    import ctypes as C
+    from asohf_amd import _lib
+    from asohf_amd.density import _GeomArgs, _ptr
+    outs = [torch.empty(snap.n, dtype=torch.float32, device="cuda:0") for _ in range(4)]
+    o = bld._opts("tsc", "deterministic", "batched", None, None)
+    st = _lib.load().asohf_hip_density_adjoint(bld._h, _GeomArgs(g).ref, snap.n, _ptr(x), _ptr(y), _ptr(z),
+                                               _ptr(m), _ptr(t1), None, *[_ptr(t) for t in outs], C.byref(o))
+    assert st != 0
+
+
 def test_adjoint_vs_central_differences_large(bld):
     spec = synth.WORKLOADS[4]
     snap = synth.make_snapshot(spec)

@@ -227,7 +227,23 @@ def test_edge_cases_empty_and_no_levels(bld):
     far.x[3] = np.float32(10 * snap.geom.lado0)
     with pytest.raises(AsohfError):
         run(bld, far, "table")
-    run(bld, far, "tsc")   # the TSC base index is clamped (grids.f:1184-1192): accepted
+    # TSC: the reference clamps the base index (grids.f:1184-1192) and its
+    # weights at the clamped centre (grids.f:1194-1213) leave [0, 1] more than
+    # half a cell past the outer centres; such positions are rejected
+    with pytest.raises(AsohfError):
+        run(bld, far, "tsc")
+    with pytest.raises(AsohfError):
+        bld.adjoint(snap.geom, *dev(far), torch.zeros((snap.geom.nz, snap.geom.ny, snap.geom.nx),
+                                                         device="cuda:0"), None)
+    # the box faces themselves (and a few ulps beyond) are accepted
+    half = np.float32(snap.geom.lado0 / 2)
+    edge = synth.Snapshot(snap.x.copy(), snap.y.copy(), snap.z, snap.m, snap.npart_esp, snap.geom)
+    edge.x[3] = -half
+    edge.y[4] = np.nextafter(half, np.float32(0))
+    edge.x[6] = np.nextafter(np.nextafter(-half, np.float32(-1)), np.float32(-1))
+    u1, _, gm = run(bld, edge, "tsc")
+    tot = float(edge.m.astype(np.float64).sum())
+    assert abs(gm[0] - tot) <= 1e-12 * tot
 
 
 def test_larger_snapshot_properties(bld):

@@ -10,7 +10,8 @@ weight products:
     are 0.5 (0.5 - t)^2, 0.75 - t^2, 0.5 (0.5 + t)^2 (grids.f:1194-1213 in
     the form the kernels evaluate), and |t| <= 1/2 up to rounding for a
     particle inside the grid or inside a patch's BUF box (the index is the
-    cell that contains it).  A non-centre product has at least one outer
+    cell that contains it); the base grid accepts positions up to 2^-10
+    cells past the box, so |t| <= 1/2 + 2^-10 there.  A non-centre product has at least one outer
     weight, so it is at most 0.5 * 0.75 * 0.75 = 0.28125.
   * Table mode: narrow entries use row 0 of COMPUTE_KERNELS' table
     (grids.f:997-1067); the plan merges only when every non-centre product of
@@ -60,6 +61,15 @@ def test_tsc_weight_products_fit_the_merge():
     assert np.all(w >= 0)
     bound = max_non_centre_product(w.max(axis=1))
     assert bound <= 0.28125 * (1 + 1e-6), bound
+    # the base grid accepts positions up to 2^-10 cells past the box
+    # (kTscBaseSlack), i.e. offsets up to 1/2 + 2^-10 from the outer centres
+    slack = f32(2.0 ** -10)
+    tw = np.concatenate([t, np.linspace(0.5, 0.5 + slack, 1001, dtype=f32),
+                         np.linspace(-0.5 - slack, -0.5, 1001, dtype=f32)])
+    ww = tsc_weights(tw)
+    assert np.all(ww >= 0)
+    bound = max(bound, max_non_centre_product(ww.max(axis=1)))
+    assert bound <= 0.2822, bound
     # four terms of a mass below 2^31 fit 32 bits; eight do not (hence the
     # third step's 64-bit add)
     q_max = np.floor(bound * 2.0 ** 31) + 1
