@@ -53,6 +53,7 @@ class SnapshotSpec:
     omega_m: float = 0.3
     side_lo: int = 8
     side_hi: int = 64
+    fdm: float = 0.845          # FDM of RENORM_DENSITY: the reference's default input (asohf.dat:11)
 
     @property
     def n(self) -> int:
@@ -326,7 +327,7 @@ def _tuned_hierarchy(spec, geom_base, halos, seed, n):
 
 def make_snapshot(spec: SnapshotSpec, n_side: int | None = None, nx: int | None = None,
                   patches_per_level: tuple | None = None, seed: int | None = None,
-                  interp_degree: int = 2, side_hi: int | None = None) -> Snapshot:
+                  interp_degree: int = 2, side_hi: int | None = None, fdm: float | None = None) -> Snapshot:
     """Build one seeded snapshot.  Optional overrides shrink a workload for tests
     (side_hi = 32 keeps every patch within the reference's NAMRX = 32)."""
     spec = dataclasses.replace(
@@ -335,7 +336,8 @@ def make_snapshot(spec: SnapshotSpec, n_side: int | None = None, nx: int | None 
         nx=nx or spec.nx,
         patches_per_level=spec.patches_per_level if patches_per_level is None else tuple(patches_per_level),
         seed=spec.seed if seed is None else seed,
-        side_hi=side_hi or spec.side_hi)
+        side_hi=side_hi or spec.side_hi,
+        fdm=spec.fdm if fdm is None else fdm)
     rng = np.random.default_rng(spec.seed)
     pos, _, halos = particles(spec, rng=rng)
     n = len(pos)
@@ -346,7 +348,7 @@ def make_snapshot(spec: SnapshotSpec, n_side: int | None = None, nx: int | None 
     npatch, pnx, pny, pnz, prx, pry, prz, pare, pc = _tuned_hierarchy(
         spec, {"nx": spec.nx, "radx": g0.radx, "dx": g0.dx}, halos, spec.seed, n)
     geom = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, npatch, pnx, pny, pnz,
-                    prx, pry, prz, pare=pare, interp_degree=interp_degree,
+                    prx, pry, prz, pare=pare, fdm=f32(spec.fdm), interp_degree=interp_degree,
                     pcx=pc[:, 0], pcy=pc[:, 1], pcz=pc[:, 2])
     x = pos[:, 0].astype(np.float32)
     y = pos[:, 1].astype(np.float32)
@@ -386,7 +388,7 @@ def make_snapshot_device(spec: SnapshotSpec, device="cuda:0", chunk: int = 1 << 
     npatch, pnx, pny, pnz, prx, pry, prz, pare, pc = _tuned_hierarchy(
         spec, {"nx": spec.nx, "radx": g0.radx, "dx": g0.dx}, (nh, conc, centre, rvir), spec.seed, n)
     geom = Geometry(spec.nx, spec.nx, spec.nx, lado0, rote, rete, npatch, pnx, pny, pnz,
-                    prx, pry, prz, pare=pare, interp_degree=interp_degree,
+                    prx, pry, prz, pare=pare, fdm=f32(spec.fdm), interp_degree=interp_degree,
                     pcx=pc[:, 0], pcy=pc[:, 1], pcz=pc[:, 2])
 
     dev = torch.device(device)

@@ -15,9 +15,11 @@ Cases:
   edges      -- hand-placed particles on base-cell faces, on the box faces
                 +-LADO0/2, on both patch-box faces of both modes (and one ulp
                 either side), and on fine-cell faces inside patches, with every
-                species present so every kernel width occurs.
+                species present so every kernel width occurs;
+  seeded_14_fdm -- 64,000 particles on a 64^3 base grid with four AMR
+                levels, divided by the reference's default FDM = 0.845.
 
-Usage: python tests/golden/make_golden.py   (needs `make -C oracle ref`)
+Usage: python tests/golden/make_golden.py [names]   (needs `make -C oracle ref`)
 """
 from __future__ import annotations
 
@@ -133,14 +135,27 @@ def edges_case():
 def main() -> None:
     if not O.ref_available():
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
+    only = set(sys.argv[1:])
+
+    def want(name):
+        return not only or name in only
     for seed in (11, 12):
-        snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=20, nx=16, patches_per_level=(3, 3, 2),
-                                   side_hi=32, seed=seed)
-        save(f"seeded_{seed}", snap)
-    snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=32, nx=32, patches_per_level=(5, 4, 3),
-                               side_hi=32, seed=13, interp_degree=1)
-    save("seeded_13_cic", snap)
-    save("edges", edges_case())
+        if want(f"seeded_{seed}"):
+            snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=20, nx=16, patches_per_level=(3, 3, 2),
+                                       side_hi=32, seed=seed, fdm=1.0)
+            save(f"seeded_{seed}", snap)
+    if want("seeded_13_cic"):
+        snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=32, nx=32, patches_per_level=(5, 4, 3),
+                                   side_hi=32, seed=13, interp_degree=1, fdm=1.0)
+        save("seeded_13_cic", snap)
+    if want("edges"):
+        save("edges", edges_case())
+    # RENORM_DENSITY at the reference's default FDM (asohf.dat:11), on a
+    # larger case: 64,000 particles, a 64^3 base grid, four AMR levels
+    if want("seeded_14_fdm"):
+        snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=40, nx=64, patches_per_level=(14, 10, 8, 5),
+                                   side_hi=32, seed=14, fdm=0.845)
+        save("seeded_14_fdm", snap)
 
 
 if __name__ == "__main__":

@@ -28,6 +28,44 @@ def test_shim_links_as_density():
     assert "libasohf_hip.so" in ldd and "not found" not in ldd.split("libasohf_hip.so")[1].split("\n")[0]
 
 
+def test_shim_replaces_density_in_the_reference_build():
+    """CPU: the reference's own main program (asohf.f, grids.f #included),
+    compiled here with its DENSITY weakened as the integration recipe says,
+    linked with the shim: the one call of DENSITY (asohf.f:467-470) then
+    resolves to the shim, and every library symbol the shim needs is exported
+    by libasohf_hip.so."""
+    import shutil
+    if not (os.path.isdir("/root/reference") and shutil.which("amdflang")):
+        pytest.skip("needs the reference tree and amdflang (build container only)")
+    odir = os.path.join(ROOT, "oracle")
+    subprocess.run(["make", "-s", "-C", odir, "dropin"], check=True, capture_output=True)
+    d = os.path.join(odir, "_ref", "dropin")
+
+    def syms(path, *flags):
+        out = subprocess.run(["nm", *flags, path], capture_output=True, text=True, check=True).stdout
+        return [ln.split() for ln in out.splitlines() if ln.strip()]
+    weak = [s for s in syms(os.path.join(d, "asohf_weak.o")) if s[-1] == "density_"]
+    assert weak and weak[0][-2] == "W"
+    both = {s[-1]: s for s in syms(os.path.join(d, "asohf_dropin.o")) if len(s) == 3}
+    assert both["density_"][1] == "T"
+    shim_addr = int(both["density_"][0], 16)
+    # the call site's relocation names the shim's definition
+    rel = subprocess.run(["readelf", "-rW", os.path.join(d, "asohf_dropin.o")], capture_output=True,
+                         text=True, check=True).stdout
+    calls = [ln.split() for ln in rel.splitlines() if ln.split()[-3:-2] == ["density_"] or
+             (len(ln.split()) >= 5 and ln.split()[4] == "density_")]
+    assert calls, "no relocation against density_"
+    for c in calls:
+        assert int(c[3], 16) == shim_addr
+    # the library entry points the shim calls are all exported
+    undefined = {s[-1] for s in syms(os.path.join(d, "asohf_dropin.o"), "-u")}
+    need = {s for s in undefined if s.startswith("asohf_hip_")}
+    assert {"asohf_hip_create", "asohf_hip_density_host"} <= need
+    lib = os.path.join(ROOT, "asohf_amd", "libasohf_hip.so")
+    exported = {s[-1] for s in syms(lib, "-D", "--defined-only")}
+    assert need <= exported
+
+
 def _write_input(path, snap):
     g = snap.geom
     with open(path, "wb") as f:
