@@ -126,6 +126,12 @@ struct HostPlan {
   std::vector<BoxRec> hrec;
   std::vector<int> hlist;    // grid id of every candidate (host copy, for the nesting check)
   std::vector<float> radx, rady, radz, ktab;
+  // TSC binning along the hierarchy (tgt_off / tgt / grec of Plan)
+  std::vector<unsigned> tgt_off;
+  std::vector<int> tgt;
+  std::vector<BoxRec> grec;
+  std::vector<int> level_tile0;   // first tile of every level, [nl + 2]
+  unsigned emit_levels = 0;       // bit l: some tile of level l has targets
   Plan dev{};
   long long ncells = 0;
   long long u11_cells = 0;   // cells of the patch output array
@@ -160,6 +166,12 @@ struct PipeArgs {
   unsigned long long cap_items;
   unsigned long long cap_split;
   int ntiles, chunk, chunk_wide, red, grid, block, cus, elem;
+  // TSC binning along the hierarchy
+  unsigned* item_pairs;
+  long long lvl_items[kNLevels + 2];   // items of level l: [lvl_items[l], lvl_items[l + 1])
+  long long n_items;
+  unsigned emit_levels;
+  int nl;
 };
 
 struct asohf_ctx {
@@ -173,12 +185,22 @@ struct asohf_ctx {
   DBuf cnt, start, cursor, bsum, counters, stiles, isplit, srank, nitems, split, tmass, entries, items, scratch;
   DBuf flags, idx, pool, pool_off, pool_nk, blk_pairs, blk_mmax, nsplit, item_off, split_off;
   DBuf elist, eoff, ecount;
+  DBuf ipairs;                             // members found per work item (TSC hierarchy binning)
   DBuf sp_k[4], sp_dens, sp_cnt, sp_tmp;   // species partition
   DBuf pk_r[2], pk_k[2], pk_tmp, pk_cnt, pk_geo;   // peak candidates
   DBuf mk_geo, mk_err, vg_cnt, vg_start, vg_pairs;  // grid masks
   long long pool_want = 0;   // pairs the last count pass needed
   long long epool_want = 0;  // emission-list words the last count pass needed
   bool sized = false;        // the entry/item/scratch buffers were sized by an exact call
+  // TSC binning along the hierarchy: per-tile capacities and the work items
+  // planned on them by an exact count, valid for one (plan, particle count,
+  // tuning); cleared when a call overflows a capacity
+  bool hier_sized = false;
+  bool hier_off = false;     // ASOHF_TSC_BINNING=hash at creation: TSC uses the patch-hash binning
+  uint64_t hier_key = 0;
+  long long hier_items[kNLevels + 2] = {};
+  long long hier_nitems = 0;
+  unsigned long long hier_nsplit = 0;
   hipGraphExec_t gexec = nullptr;   // the captured batched build and the arguments it holds
   PipeArgs gkey;
   DBuf h_in[4], h_out[2], h_mass;    // device copies for the host entry point
@@ -315,7 +337,10 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
       B.org[a] = -g->lado0 / 2.0f;
     }
     B.d[0] = g->dx; B.d[1] = g->dy; B.d[2] = g->dz;
-    for (int a = 0; a < 3; ++a) B.inv_d[a] = 1.0f / B.d[a];
+    for (int a = 0; a < 3; ++a) {
+      B.inv_d[a] = 1.0f / B.d[a];
+      B.inv_dd[a] = 1.0 / (double)B.d[a];
+    }
     B.den = g->dx * g->dy * g->dz * g->rote * r3;         // DENBAS (grids.f:824)
     B.tile0 = 0;
     ntiles = B.nt[0] * B.nt[1] * B.nt[2];
@@ -336,7 +361,10 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
       G.n[0] = g->patchnx[p]; G.n[1] = g->patchny[p]; G.n[2] = g->patchnz[p];
       G.level = l;
       G.d[0] = dxpa; G.d[1] = dypa; G.d[2] = dzpa;
-      for (int a = 0; a < 3; ++a) G.inv_d[a] = 1.0f / G.d[a];
+      for (int a = 0; a < 3; ++a) {
+        G.inv_d[a] = 1.0f / G.d[a];
+        G.inv_dd[a] = 1.0 / (double)G.d[a];
+      }
       G.den = den;
       const float pr[3] = {g->patchrx[p], g->patchry[p], g->patchrz[p]};
       const float d[3] = {dxpa, dypa, dzpa};
@@ -376,6 +404,27 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
       const long long nt = (long long)G.nt[0] * G.nt[1] * G.nt[2];
       if ((long long)ntiles + nt > (1ll << 30)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^30 tiles");
       ntiles += (int)nt;
+    }
+  }
+  // TSC weights are taken from the reference's three FP32 centres of the
+  // stencil, each rounded on its own; their spacing differs from the cell
+  // size by up to two ulps of the coordinate.  Up to a quarter of a cell every
+  // weight stays non-negative and every non-centre product below 1/2 (the
+  // exact deposit's term bounds); beyond it the grid is too fine for FP32
+  // coordinates at its position and the build refuses it.
+  if (mode == ASOHF_MODE_TSC) {
+    for (size_t gi = 0; gi < H.grids.size(); ++gi) {
+      const GridDesc& G = H.grids[gi];
+      if (gi > 0 && G.level == 0) continue;
+      for (int a = 0; a < 3; ++a) {
+        const double lo = gi == 0 ? -0.5 * g->lado0 - G.d[a] : G.lo[a] - G.d[a];
+        const double hi = gi == 0 ? 0.5 * g->lado0 + G.d[a] : G.hi[a] + G.d[a];
+        const double mx = std::max(std::fabs(lo), std::fabs(hi));
+        const double ulp = mx > 0 ? std::ldexp(1.0, std::ilogb(mx) - 23) : 0.0;
+        if (2.0 * ulp > 0.25 * (double)G.d[a])
+          return fail(ctx, ASOHF_ERR_CAPACITY,
+                      "a grid's cells are too small for FP32 coordinates at its position (TSC centres)");
+      }
     }
   }
   H.u11_cells = padded ? P * (long long)o->u11_ld[0] * o->u11_ld[1] * o->u11_ld[2] : u11_off;
@@ -482,12 +531,12 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
     }
   }
   if (H.hlist.empty()) H.hlist.push_back(0);
-  H.hrec.resize(H.hlist.size());
-  for (size_t i = 0; i < H.hlist.size(); ++i) {
-    const GridDesc& G = H.grids[H.hlist[i]];
-    BoxRec& r = H.hrec[i];
+  auto make_rec = [&](int gi) {
+    const GridDesc& G = H.grids[gi];
+    BoxRec r;
+    memset(&r, 0, sizeof(r));
     for (int a = 0; a < 3; ++a) { r.lo[a] = G.lo[a]; r.hi[a] = G.hi[a]; }
-    r.g = H.hlist[i];
+    r.g = gi;
     r.level = G.level;
     for (int a = 0; a < 3; ++a) {
       r.org[a] = G.org[a];
@@ -504,6 +553,91 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
     r.ld2 = G.ld2;
     r.pad = 0;
     r.off = G.off;
+    return r;
+  };
+  H.hrec.resize(H.hlist.size());
+  for (size_t i = 0; i < H.hlist.size(); ++i) H.hrec[i] = make_rec(H.hlist[i]);
+  // ---- first tile of every level (grids, and so tiles, are numbered level
+  // by level; patches listed at level 0 have no tiles) ----
+  H.level_tile0.assign(nl + 2, ntiles);
+  for (int l = nl; l >= 1; --l)
+    H.level_tile0[l] = g->npatch[l] > 0 ? H.grids[1 + level_start[l]].tile0 : H.level_tile0[l + 1];
+  H.level_tile0[0] = 0;
+  // ---- TSC binning along the hierarchy: every level-l patch (l >= 1) gets an
+  // emitter, the deepest grid found whose membership box holds the patch's box
+  // in FP32 (so every member of the patch is a member of the emitter): a
+  // patch of the level above whose box holds it, else one two levels up, ...,
+  // else the base grid, which holds every particle.  The patch is then a
+  // target of every emitter tile whose home region (the positions whose
+  // clamped cell lies in the tile) can meet its box. ----
+  if (mode == ASOHF_MODE_TSC) {
+    H.grec.resize(H.grids.size());
+    for (size_t gi = 0; gi < H.grids.size(); ++gi) H.grec[gi] = make_rec((int)gi);
+    auto contains = [&](const GridDesc& Q, const GridDesc& Cg) {
+      return Q.lo[0] <= Cg.lo[0] && Cg.hi[0] <= Q.hi[0] && Q.lo[1] <= Cg.lo[1] && Cg.hi[1] <= Q.hi[1] &&
+             Q.lo[2] <= Cg.lo[2] && Cg.hi[2] <= Q.hi[2];
+    };
+    auto find_container = [&](int lq, const GridDesc& Cg) -> int {
+      const LevelHash& lh = D.lh[lq];
+      int b[3];
+      for (int a = 0; a < 3; ++a) {
+        const double mid = 0.5 * ((double)Cg.lo[a] + (double)Cg.hi[a]);
+        double t = (mid - lh.x0) * (double)lh.inv_bs;
+        t = std::min(std::max(t, -1.0), 1.0e6);
+        b[a] = std::min(std::max((int)std::floor(t), 0), lh.nb - 1);
+      }
+      const uint64_t key = bucket_key(lq, b[0], b[1], b[2]);
+      auto itk = std::lower_bound(ukeys.begin(), ukeys.end(), key);
+      if (itk == ukeys.end() || *itk != key) return -1;
+      const int2 se = uvals[itk - ukeys.begin()];
+      for (int q = 0; q < se.y; ++q) {
+        const int gq = H.hlist[se.x + q];
+        if (contains(H.grids[gq], Cg)) return gq;
+      }
+      return -1;
+    };
+    std::vector<std::pair<int, int>> tp;   // (emitter tile, target grid)
+    for (int l = 1; l <= nl; ++l) {
+      for (int p = level_start[l]; p < level_start[l + 1]; ++p) {
+        const int gc = 1 + p;
+        const GridDesc& Cg = H.grids[gc];
+        int ge = 0;
+        for (int lq = l - 1; lq >= 1 && ge == 0; --lq) {
+          const int f = find_container(lq, Cg);
+          if (f > 0) ge = f;
+        }
+        const GridDesc& E = H.grids[ge];
+        H.emit_levels |= 1u << E.level;
+        int t0[3], t1[3];
+        for (int a = 0; a < 3; ++a) {
+          // cells of E whose home region can meet (lo, hi), one cell of margin
+          const double u0 = ((double)Cg.lo[a] - (double)E.org[a]) / (double)E.d[a];
+          const double u1 = ((double)Cg.hi[a] - (double)E.org[a]) / (double)E.d[a];
+          const double c0 = std::floor(std::min(std::max(u0, -4.0), (double)E.n[a] + 4.0)) + (ge == 0 ? 0.0 : -1.0);
+          const double c1 = std::floor(std::min(std::max(u1, -4.0), (double)E.n[a] + 4.0)) + (ge == 0 ? 2.0 : 1.0);
+          const int i0 = std::min(std::max((int)c0, 1), E.n[a]);
+          const int i1 = std::min(std::max((int)c1, 1), E.n[a]);
+          t0[a] = (i0 - 1) / T[a];
+          t1[a] = (i1 - 1) / T[a];
+        }
+        for (int c = t0[2]; c <= t1[2]; ++c)
+          for (int b = t0[1]; b <= t1[1]; ++b)
+            for (int a = t0[0]; a <= t1[0]; ++a)
+              tp.emplace_back(E.tile0 + (c * E.nt[1] + b) * E.nt[0] + a, gc);
+      }
+    }
+    H.tgt_off.assign(ntiles + 1, 0u);
+    for (const auto& e : tp) H.tgt_off[e.first + 1]++;
+    for (int t = 0; t < ntiles; ++t) H.tgt_off[t + 1] += H.tgt_off[t];
+    H.tgt.assign(std::max<size_t>(tp.size(), 1), 0);
+    {
+      std::vector<unsigned> fill(H.tgt_off.begin(), H.tgt_off.end() - 1);
+      for (const auto& e : tp) H.tgt[fill[e.first]++] = e.second;
+    }
+  } else {
+    H.tgt_off.assign(ntiles + 1, 0u);
+    H.tgt.assign(1, 0);
+    H.grec.assign(1, BoxRec{});
   }
   // ---- upload ----
   size_t off = 0;
@@ -516,6 +650,9 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   const size_t o_ry = place(sizeof(float) * H.rady.size());
   const size_t o_rz = place(sizeof(float) * H.radz.size());
   const size_t o_kt = place(sizeof(float) * H.ktab.size());
+  const size_t o_to = place(sizeof(unsigned) * H.tgt_off.size());
+  const size_t o_tt = place(sizeof(int) * H.tgt.size());
+  const size_t o_gr = place(sizeof(BoxRec) * H.grec.size());
   HIPCHK(ctx->plan_blob.ensure(off), "allocating the geometry plan");
   char* base = ctx->plan_blob.as<char>();
   auto up = [&](size_t o2, const void* src, size_t bytes) {
@@ -529,6 +666,9 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   HIPCHK(up(o_ry, H.rady.data(), sizeof(float) * H.rady.size()), "uploading RADY");
   HIPCHK(up(o_rz, H.radz.data(), sizeof(float) * H.radz.size()), "uploading RADZ");
   HIPCHK(up(o_kt, H.ktab.data(), sizeof(float) * H.ktab.size()), "uploading kernel table");
+  HIPCHK(up(o_to, H.tgt_off.data(), sizeof(unsigned) * H.tgt_off.size()), "uploading targets");
+  HIPCHK(up(o_tt, H.tgt.data(), sizeof(int) * H.tgt.size()), "uploading targets");
+  HIPCHK(up(o_gr, H.grec.data(), sizeof(BoxRec) * H.grec.size()), "uploading grid records");
   HIPCHK(hipStreamSynchronize(ctx->stream), "uploading the plan");
   D.grids = (const GridDesc*)(base + o_grids);
   D.tile_grid = (const int*)(base + o_tg);
@@ -538,6 +678,9 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   D.rady = (const float*)(base + o_ry);
   D.radz = (const float*)(base + o_rz);
   D.ktab = (const float*)(base + o_kt);
+  D.tgt_off = (const unsigned*)(base + o_to);
+  D.tgt = (const int*)(base + o_tt);
+  D.grec = (const BoxRec*)(base + o_gr);
   D.nl = nl;
   D.ngrids = (int)H.grids.size();
   D.ntiles = ntiles;
@@ -547,8 +690,8 @@ static asohf_status build_plan(asohf_ctx* ctx, const asohf_geom* g, int mode, co
   for (int a = 0; a < 3; ++a) { D.tile[a] = T[a]; D.tshift[a] = __builtin_ctz(T[a]); }
   D.mode = mode;
   // lane merging of narrow deposits sums up to four non-centre terms in 32
-  // bits: TSC weight products are at most 0.5 * 0.75 * 0.75; the table's row
-  // 0 (centred at kmax) is checked here
+  // bits: TSC non-centre weight products stay below 1/2 (the centre check
+  // above); the table's row 0 (centred at kmax) is checked here
   {
     const float* k0 = H.ktab.data() + D.kmax - 1;
     float mx = 0.f;
@@ -640,7 +783,7 @@ static asohf_status decode_err(asohf_ctx* ctx, unsigned e) {
 // buffers keep the size of the largest earlier call; the gate kernel checks
 // them on the device.
 static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, float* u1, float* u11,
-                                 double* grid_mass, PipeArgs& a) {
+                                 double* grid_mass, PipeArgs& a, bool want_elist = true) {
   HostPlan& H = ctx->plan;
   memset(&a, 0, sizeof(a));
   a.P = H.dev;
@@ -675,7 +818,7 @@ static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Part
   HIPCHK(ctx->blk_pairs.ensure(sizeof(unsigned long long) * std::max<long long>(nblk, 1)), "allocating block sums");
   HIPCHK(ctx->blk_mmax.ensure(sizeof(unsigned) * std::max<long long>(nblk, 1)), "allocating block sums");
   // emission lists: about four per particle until a call has measured them
-  a.BT.use_elist = 2ll * ntiles < (1ll << 23) ? 1 : 0;
+  a.BT.use_elist = want_elist && 2ll * ntiles < (1ll << 23) ? 1 : 0;
   if (a.BT.use_elist) {
     const long long ecap = std::max<long long>(ctx->epool_want + ctx->epool_want / 8, 4 * A.n + 4096);
     HIPCHK(ctx->elist.ensure(sizeof(unsigned) * ecap), "allocating emission lists");
@@ -707,10 +850,11 @@ static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Part
   a.split_tiles = ctx->stiles.as<int>();
   a.grid_mass = grid_mass;
   a.chunk = std::max(ctx->tune.chunk, 64);
-  // wide table-mode entries carry up to 129^3 cells each against a narrow
-  // entry's 27; chunk/8 of them per item measured best on the 134M-particle
-  // workload (chunk/64: 195 ms per build, chunk/16: 180, chunk/8: 177)
-  a.chunk_wide = std::max(a.chunk / 8, 4);
+  // wide table-mode entries: a tile's classes are deposited in the
+  // histogram form (a fixed cost per tile and class) when one work item
+  // holds all of the tile's wide entries, otherwise (and for the widest
+  // classes) entry by entry; items of up to 8 chunks of entries
+  a.chunk_wide = a.chunk * 8;
   a.cus = ctx->cus;
   const int waves = std::max(ctx->tune.waves_per_cu, 1);
   a.block = ctx->tune.block;
@@ -840,6 +984,186 @@ static asohf_status run_exact(asohf_ctx* ctx, const asohf_opts& o, const Parts& 
   return ASOHF_OK;
 }
 
+// ---------------------------------------------------------------------------
+// TSC: binning along the hierarchy (hier.hip)
+// ---------------------------------------------------------------------------
+static uint64_t hier_key(const asohf_ctx* ctx, const Parts& A, int red) {
+  uint64_t h = ctx->plan.key;
+  const long long v[6] = {A.n, ctx->tune.chunk, ctx->tune.tile[0], ctx->tune.tile[1], ctx->tune.tile[2], red};
+  return fnv(h, v, sizeof(v));
+}
+
+// Exact form of the capacities: the count pass (every particle enumerated
+// over the patch hash), a capacity per tile from its count, offsets, the work
+// items planned on the capacities, and the buffers sized to them.  The items
+// stay valid for later calls on the same plan, particle count and tuning.
+static asohf_status hier_plan(asohf_ctx* ctx, PipeArgs& a, hipStream_t s) {
+  HostPlan& H = ctx->plan;
+  HIPCHK(launch_fill_u32((unsigned*)a.C, 0u, sizeof(Counters) / 4, s), "clearing counters");
+  HIPCHK(launch_fill_u32(a.cnt, 0u, a.nkeys, s), "clearing counts");
+  HIPCHK(launch_count(a.P, a.A, a.cnt, a.C, a.BT, a.cus, s), "count kernel");
+  HIPCHK(launch_count_to_capacity(a.cnt, a.nkeys, a.cus, s), "capacities");
+  HIPCHK(launch_scan(a.cnt, a.nkeys, a.bsum, a.start, a.cursor, s), "scan");
+  HIPCHK(launch_plan_count(a.start, a.ntiles, a.chunk, a.chunk_wide, a.nitems, a.nsplit, a.is_split, s),
+         "planning items");
+  HIPCHK(launch_scan(a.nitems, a.ntiles, a.bsum, a.item_off, nullptr, s), "item offsets");
+  HIPCHK(launch_scan(a.nsplit, a.ntiles, a.bsum, a.split_off, nullptr, s), "split offsets");
+  HIPCHK(launch_scan(a.is_split, a.ntiles, a.bsum, a.split_rank, nullptr, s), "split tiles");
+  const int nl = a.P.nl;
+  HIPCHK(hipMemcpyAsync(ctx->hcount, a.C, sizeof(Counters), hipMemcpyDeviceToHost, s), "reading counters");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar, a.start + a.nkeys, sizeof(unsigned long long), hipMemcpyDeviceToHost, s),
+         "reading totals");
+  HIPCHK(hipMemcpyAsync(ctx->hscalar + 1, a.split_off + a.ntiles, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, s), "reading totals");
+  for (int l = 0; l <= nl + 1; ++l)
+    HIPCHK(hipMemcpyAsync(ctx->hscalar + 2 + l, a.item_off + H.level_tile0[l], sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s), "reading item offsets");
+  HIPCHK(hipStreamSynchronize(s), "count pass");
+  asohf_status st = decode_err(ctx, ctx->hcount->err);
+  if (st != ASOHF_OK) return st;
+  const unsigned long long cap_entries = ctx->hscalar[0];
+  const unsigned long long nsplit = ctx->hscalar[1];
+  const long long n_items = (long long)ctx->hscalar[2 + nl + 1];
+  if (n_items >= (1ll << 31)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^31 work items");
+  for (int l = 0; l <= nl + 1; ++l) ctx->hier_items[l] = (long long)ctx->hscalar[2 + l];
+  const int T = a.P.tile[0] * a.P.tile[1] * a.P.tile[2];
+  HIPCHK(ctx->entries.ensure(sizeof(int4) * std::max<unsigned long long>(cap_entries, 1)), "allocating entries");
+  HIPCHK(ctx->items.ensure(sizeof(Item) * std::max<long long>(n_items, 1)), "allocating work items");
+  HIPCHK(ctx->scratch.ensure((size_t)std::max<unsigned long long>(nsplit, 1) * T * a.elem),
+         "allocating split-tile scratch");
+  HIPCHK(ctx->ipairs.ensure(sizeof(unsigned) * std::max<long long>(n_items, 1)), "allocating item sums");
+  a.D.entries = ctx->entries.as<int4>();
+  a.D.items = ctx->items.as<Item>();
+  a.D.scratch = ctx->scratch.p;
+  HIPCHK(launch_plan_write(a.start, a.ntiles, a.chunk, a.chunk_wide, a.item_off, a.split_off, a.split_rank,
+                           const_cast<Item*>(a.D.items), a.split, a.split_tiles, a.D.tile_mass_i, a.C, s),
+         "writing items");
+  ctx->hier_nitems = n_items;
+  ctx->hier_nsplit = nsplit;
+  return ASOHF_OK;
+}
+
+// the arguments of a call on the planned capacities
+static void hier_bind(asohf_ctx* ctx, PipeArgs& a) {
+  a.D.entries = ctx->entries.as<int4>();
+  a.D.items = ctx->items.as<Item>();
+  a.D.scratch = ctx->scratch.p;
+  a.D.cursor = a.cursor;
+  a.item_pairs = ctx->ipairs.as<unsigned>();
+  a.n_items = ctx->hier_nitems;
+  for (int l = 0; l <= kNLevels + 1; ++l) a.lvl_items[l] = ctx->hier_items[std::min(l, a.P.nl + 1)];
+  a.emit_levels = ctx->plan.emit_levels;
+  a.nl = a.P.nl;
+  a.grid = (int)std::min<long long>(a.grid, std::max<long long>(a.n_items, 1));
+  a.D.n_queues = std::min(kQueues, a.grid);
+}
+
+// One call on the planned capacities: base-grid binning, one emission pass
+// per level, the persistent deposit over every item, the split tiles, the
+// grid sums and the call's statistics; no host round trip (graph-capturable).
+static asohf_status enqueue_hier(asohf_ctx* ctx, const PipeArgs& a, hipStream_t s, bool events) {
+  HIPCHK(launch_fill_u32((unsigned*)a.C, 0u, sizeof(Counters) / 4, s), "clearing counters");
+  HIPCHK(launch_fill_u32(&a.C->n_items_eff, (unsigned)a.n_items, 1, s), "queue length");
+  HIPCHK(launch_fill_u32(a.item_pairs, 0u, a.n_items, s), "clearing item sums");
+  HIPCHK(launch_cursor_reset(a.start, a.cursor, a.nkeys, a.cus, s), "cursors");
+  HIPCHK(launch_bin_base_tsc(a.P, a.A, a.cursor, a.start, const_cast<int4*>(a.D.entries), a.C, a.BT, s),
+         "base-grid binning");
+  if (a.nblk > 0) HIPCHK(launch_reduce_blocks(a.BT, a.nblk, a.C, s), "block sums");
+  if (events) {
+    HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
+    HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
+  }
+  for (int l = 0; l < a.nl; ++l) {
+    if (!(a.emit_levels & (1u << l))) continue;
+    HIPCHK(launch_emit_tsc(a.P, a.D.items, a.lvl_items[l], a.lvl_items[l + 1], a.cursor, a.start,
+                           const_cast<int4*>(a.D.entries), a.item_pairs, a.C, a.cus, s),
+           "emission pass");
+  }
+  if (events) HIPCHK(hipEventRecord(ctx->ev[3], s), "event");
+  HIPCHK(launch_zero_split(a.P, a.D, a.red, a.cus, s), "clearing split tiles");
+  if (events) HIPCHK(hipEventRecord(ctx->ev[4], s), "event");
+  HIPCHK(launch_deposit(a.P, a.D, a.red, a.grid, a.block, s), "deposit kernel");
+  if (events) HIPCHK(hipEventRecord(ctx->ev[8], s), "event");
+  HIPCHK(launch_split_reduce(a.P, a.D, a.red, a.cus, s), "split tiles");
+  if (events) HIPCHK(hipEventRecord(ctx->ev[5], s), "event");
+  if (a.grid_mass) HIPCHK(launch_grid_sums(a.P, a.D, a.red, a.grid_mass, s), "grid sums");
+  HIPCHK(launch_hier_stats(a.item_pairs, a.n_items, a.start, a.cursor, a.nkeys, a.C, s), "statistics");
+  return ASOHF_OK;
+}
+
+static asohf_status run_hier(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, float* u1, float* u11,
+                             double* grid_mass, hipStream_t s) {
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    PipeArgs a;
+    // the count pass keeps no emission lists here
+    asohf_status st = prepare_args(ctx, o, A, u1, u11, grid_mass, a, false);
+    if (st != ASOHF_OK) return st;
+    const uint64_t hk = hier_key(ctx, A, o.reduction);
+    bool planned = false;
+    if (!ctx->hier_sized || ctx->hier_key != hk) {
+      ctx->hier_sized = false;
+      if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+      ctx->gexec = nullptr;
+      HIPCHK(hipEventRecord(ctx->ev[0], s), "event");
+      st = hier_plan(ctx, a, s);
+      if (st != ASOHF_OK) return st;
+      ctx->hier_key = hk;
+      ctx->hier_sized = true;
+      planned = true;
+    }
+    hier_bind(ctx, a);
+    hipStream_t rs = s;
+    bool phases = true;
+    if (o.use_graph && !planned) {
+      phases = false;
+      rs = ctx->stream;
+      if (!ctx->gexec || memcmp(&ctx->gkey, &a, sizeof(a)) != 0) {
+        if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
+        ctx->gexec = nullptr;
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(rs, hipStreamCaptureModeThreadLocal), "graph capture");
+        st = enqueue_hier(ctx, a, rs, false);
+        const hipError_t e = hipStreamEndCapture(rs, &graph);
+        if (st != ASOHF_OK) {
+          if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+          return st;
+        }
+        HIPCHK(e, "graph capture");
+        const hipError_t ei = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) ctx->gexec = nullptr;
+        HIPCHK(ei, "instantiating the graph");
+        ctx->gkey = a;
+      }
+      if (s != rs) {
+        HIPCHK(hipEventRecord(ctx->ev[7], s), "event");
+        HIPCHK(hipStreamWaitEvent(rs, ctx->ev[7], 0), "stream wait");
+      }
+      HIPCHK(hipEventRecord(ctx->ev[0], rs), "event");
+      HIPCHK(hipGraphLaunch(ctx->gexec, rs), "graph launch");
+    } else {
+      if (!planned) HIPCHK(hipEventRecord(ctx->ev[0], rs), "event");
+      st = enqueue_hier(ctx, a, rs, true);
+      if (st != ASOHF_OK) return st;
+    }
+    HIPCHK(hipEventRecord(ctx->ev[6], rs), "event");
+    HIPCHK(hipMemcpyAsync(ctx->hcount, a.C, sizeof(Counters), hipMemcpyDeviceToHost, rs), "reading counters");
+    HIPCHK(hipStreamSynchronize(rs), "batched build");
+    Counters hc = *ctx->hcount;
+    st = decode_err(ctx, hc.err);
+    if (st != ASOHF_OK) return st;
+    if (hc.overflow) {
+      // a tile received more entries than its capacity: plan them again
+      ctx->hier_sized = false;
+      continue;
+    }
+    hc.n_items = (unsigned long long)a.n_items;
+    set_stats(ctx, A, hc, phases);
+    return ASOHF_OK;
+  }
+  return fail(ctx, ASOHF_ERR_CAPACITY, "tile capacities overflowed right after an exact count");
+}
+
 // Batched build.  Once a call has sized the buffers, later calls run without
 // any host round trip before the deposit (the gate kernel checks capacities
 // on the device) and read the counters once at the end; with opts.use_graph
@@ -848,6 +1172,8 @@ static asohf_status run_exact(asohf_ctx* ctx, const asohf_opts& o, const Parts& 
 static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, float* u1, float* u11,
                                 double* grid_mass, hipStream_t s) {
   if (A.n > (1ll << 32)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^32 particles");
+  if (o.mode == ASOHF_MODE_TSC && !ctx->hier_off) return run_hier(ctx, o, A, u1, u11, grid_mass, s);
+  ctx->hier_sized = false;   // this path re-plans the offsets and items the hierarchy binning keeps
   if (!ctx->sized) return run_exact(ctx, o, A, u1, u11, grid_mass, s);
   PipeArgs a;
   asohf_status st = prepare_args(ctx, o, A, u1, u11, grid_mass, a);
@@ -913,6 +1239,7 @@ static asohf_status run_batched(asohf_ctx* ctx, const asohf_opts& o, const Parts
 static asohf_status run_plain(asohf_ctx* ctx, const asohf_opts& o, const Parts& A, const asohf_geom* g,
                               float* u1, float* u11, double* grid_mass, hipStream_t s) {
   (void)o;
+  ctx->hier_sized = false;   // the offsets buffer is reused below
   const Plan& P = ctx->plan.dev;
   HIPCHK(ctx->counters.ensure(sizeof(Counters)), "allocating counters");
   Counters* C = ctx->counters.as<Counters>();
@@ -984,11 +1311,17 @@ asohf_status asohf_hip_create(asohf_ctx** out, int32_t device) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) { delete ctx; return ASOHF_ERR_HIP; }
   ctx->cus = prop.multiProcessorCount;
+  {
+    // ASOHF_TSC_BINNING=hash: the TSC build bins with the patch-hash count
+    // and scatter passes (the kernel-table mode always does)
+    const char* b = getenv("ASOHF_TSC_BINNING");
+    ctx->hier_off = b && strcmp(b, "hash") == 0;
+  }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return ASOHF_ERR_HIP; }
   for (auto& e : ctx->ev)
     if (hipEventCreate(&e) != hipSuccess) { asohf_hip_destroy(ctx); return ASOHF_ERR_HIP; }
   if (hipHostMalloc((void**)&ctx->hcount, sizeof(Counters)) != hipSuccess ||
-      hipHostMalloc((void**)&ctx->hscalar, sizeof(unsigned long long) * 4) != hipSuccess) {
+      hipHostMalloc((void**)&ctx->hscalar, sizeof(unsigned long long) * 16) != hipSuccess) {
     asohf_hip_destroy(ctx);
     return ASOHF_ERR_OOM;
   }
@@ -1178,7 +1511,7 @@ asohf_status asohf_hip_membership(asohf_ctx* ctx, const asohf_geom* geom, int64_
   HIPCHK(ctx->counters.ensure(sizeof(Counters)), "allocating counters");
   unsigned long long* c = (unsigned long long*)ctx->counters.p;
   hipStream_t s = ctx->stream;
-  HIPCHK(launch_fill_u32((unsigned*)c, 0u, 2, s), "clearing");
+  HIPCHK(launch_fill_u32((unsigned*)c, 0u, 4, s), "clearing");   // the count and the validation flags
   HIPCHK(launch_membership(ctx->plan.dev, A, out, cap, c, ctx->cus, s), "membership kernel");
   HIPCHK(hipMemcpyAsync(ctx->hscalar, c, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s), "D2H");
   HIPCHK(hipStreamSynchronize(s), "membership");
@@ -1929,7 +2262,9 @@ asohf_status asohf_hip_release_workspaces(asohf_ctx* ctx) {
                   &ctx->pk_cnt, &ctx->pk_geo, &ctx->mk_geo, &ctx->mk_err, &ctx->vg_cnt, &ctx->vg_start,
                   &ctx->vg_pairs};
   for (DBuf* b : bufs) b->release();
+  ctx->ipairs.release();
   ctx->sized = false;   // the next batched call sizes its buffers again
+  ctx->hier_sized = false;
   return ASOHF_OK;
 }
 

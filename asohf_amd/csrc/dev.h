@@ -16,6 +16,44 @@ namespace asohf {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// ---------------------------------------------------------------------------
+// wave / block reductions
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ unsigned wave_or(unsigned v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// fixed-shape block sum (deterministic for a given blockDim)
+template <class T>
+__device__ T block_sum(T v, T* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  v = wave_sum(v);
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < nw; ++k) r += sh[k];
+    sh[0] = r;
+  }
+  __syncthreads();
+  r = sh[0];
+  __syncthreads();
+  return r;
+}
+
 // single periodic wrap of grids.f:808-813 (the TSC wrap of grids.f:1221-1226
 // is the same map on the clamped index range)
 __device__ __forceinline__ int wrap1(int i, int n) {
@@ -31,24 +69,42 @@ __device__ __forceinline__ void tsc_weights(float xp, float c0, float c1, float 
   b = fabsf(xp - c1) / d; w[1] = 0.75f - b * b;
   b = fabsf(xp - c2) / d; t = 1.5f - b; w[2] = 0.5f * (t * t);
 }
-// The same three weights from t = (xp - RAD(I)) / DX alone: for a particle of
-// the cell (|t| <= 1/2 up to rounding) |xp - RAD(I-1)|/DX = 1 + t and
-// |xp - RAD(I+1)|/DX = 1 - t, so the reference's expressions become these
-// polynomials (equal in exact arithmetic, within a few ulp in FP32).
-__device__ __forceinline__ void tsc_weights_t(float t, float w[3]) {
-  const float a = 0.5f - t, b = 0.5f + t;
-  w[0] = 0.5f * (a * a);
-  w[1] = 0.75f - t * t;
-  w[2] = 0.5f * (b * b);
+// The same weights with the reciprocal of the cell size: each b within 1.5
+// ulp of the reference's quotient.  The three centres are the reference's
+// own FP32 values (RADX(I-1..I+1), MINIRADX(I-1..I+1)): each is rounded on
+// its own, and at large coordinates their spacing differs from the cell size
+// by a noticeable fraction of a cell (ulp(250) is 0.4% of a 1/128 cell of a
+// 500 Mpc/h, 1024-cell grid), which the weights must see as the reference
+// does.
+__device__ __forceinline__ void tsc_weights_c(float xp, float c0, float c1, float c2, float inv_d, float w[3]) {
+  float b, t;
+  b = fabsf(xp - c0) * inv_d; t = 1.5f - b; w[0] = 0.5f * (t * t);
+  b = fabsf(xp - c1) * inv_d; w[1] = 0.75f - b * b;
+  b = fabsf(xp - c2) * inv_d; t = 1.5f - b; w[2] = 0.5f * (t * t);
 }
-// derivatives of the three weights with respect to xp
-__device__ __forceinline__ void tsc_dweights(float xp, float c0, float c1, float c2, float d, float dw[3]) {
+// The deposit's form: distances to the three centres in FP64 (exact
+// differences of FP32 values), the weights rounded once to FP32.  The
+// reference's FP32 evaluation cancels in 1.5 - b as b approaches 3/2, where
+// a corner weight is tiny and its FP32 value keeps few correct digits; in
+// FP64 every weight is within one FP32 rounding of the exact value on the
+// reference's operands, so even a cell reached only by such a corner holds
+// its FP64 value to FP32 accuracy.
+__device__ __forceinline__ void tsc_weights_f64(float xp, float c0, float c1, float c2, double inv_d, float w[3]) {
+  const double x = (double)xp;
+  double b, t;
+  b = fabs(x - (double)c0) * inv_d; t = 1.5 - b; w[0] = (float)(0.5 * (t * t));
+  b = fabs(x - (double)c1) * inv_d; w[1] = (float)(0.75 - b * b);
+  b = fabs(x - (double)c2) * inv_d; t = 1.5 - b; w[2] = (float)(0.5 * (t * t));
+}
+// and their derivatives with respect to xp
+__device__ __forceinline__ void tsc_weights_dc(float xp, float c0, float c1, float c2, float inv_d, float w[3],
+                                               float dw[3]) {
+  tsc_weights_c(xp, c0, c1, c2, inv_d, w);
   float t;
-  t = xp - c0; dw[0] = -(1.5f - fabsf(t) / d) * (t > 0.f ? 1.f : (t < 0.f ? -1.f : 0.f)) / d;
-  t = xp - c1; dw[1] = -2.0f * t / (d * d);
-  t = xp - c2; dw[2] = -(1.5f - fabsf(t) / d) * (t > 0.f ? 1.f : (t < 0.f ? -1.f : 0.f)) / d;
+  t = xp - c0; dw[0] = -(1.5f - fabsf(t) * inv_d) * (t > 0.f ? 1.f : (t < 0.f ? -1.f : 0.f)) * inv_d;
+  t = xp - c1; dw[1] = -2.0f * t * (inv_d * inv_d);
+  t = xp - c2; dw[2] = -(1.5f - fabsf(t) * inv_d) * (t > 0.f ? 1.f : (t < 0.f ? -1.f : 0.f)) * inv_d;
 }
-
 // The quotient a/d whose integer part is a cell index, exact where it
 // matters: t = a*(1/d) is within |q|*1.2e-7 of q = a/d and RN(q) within
 // |q|*6e-8, so when t is farther than |t|*2^-20 from every integer, RN(q) lies
@@ -83,7 +139,7 @@ __device__ __forceinline__ int tsc_base_index(float xp, float xl, float d, float
 // rejects positions outside its wrap range.  The slack of 2^-10 cells keeps
 // every position of the box (whose quotient may round a few ulps past 0 or
 // N) and bounds |offset| by 1/2 + 2^-10, where every non-centre weight
-// product stays below 0.2822.
+// product stays below 0.2822 (below 1/2 with the centres' own rounding).
 constexpr float kTscBaseSlack = 9.765625e-4f;
 __device__ __forceinline__ bool tsc_base_in_range(float xp, float xl, float inv_d, int n) {
   const float t = (xp - xl) * inv_d;

@@ -30,6 +30,7 @@ struct GridDesc {
   int ld1;           // output row stride (elements)
   int ld2;           // output plane stride (elements)
   long long off;     // output offset of cell (1,1,1) in u1 (g = 0) or u11
+  double inv_dd[3];  // 1/d in FP64 (TSC weights of the batched deposit)
 };
 
 // Slot of the per-level spatial hash: bucket key and its candidate range, in
@@ -99,6 +100,12 @@ struct Plan {
                              // <= 0.45, so four terms of a mass below 2^31 sum below 2^32)
   float rad1[3];
   float radc[3];
+  // TSC binning along the hierarchy: the patches whose members are found
+  // from tile t's home entries are tgt[tgt_off[t] .. tgt_off[t+1]) (grid
+  // ids); grec[g] is grid g's box and fields in candidate-record form
+  const unsigned* tgt_off;   // [ntiles + 1]
+  const int* tgt;
+  const BoxRec* grec;        // [1 + P]
 };
 
 // Particle inputs of one call.
@@ -139,7 +146,8 @@ static_assert(sizeof(unsigned long long) * 4 + sizeof(unsigned) * (kLine - 8) ==
 // Work item of the persistent deposit: a chunk of one tile's entry list.
 struct Item {
   int tile;
-  int flags;                 // bit 0: wide (table-mode N >= 2) entries
+  int flags;                 // bit 0: wide (table-mode N >= 2) entries; bit 1: every wide entry
+                             // of the tile; bits 8+: the item's rank within its tile
   long long begin;           // entry range [begin, end)
   long long end;
 };
@@ -179,6 +187,12 @@ struct DepArgs {
   long long* tile_mass_i;    // deterministic: exact fixed-point mass per tile
   double* tile_mass_d;       // atomic: FP64 mass per tile
   int n_queues;              // queues in use (<= kQueues, <= grid)
+  // TSC binning along the hierarchy: items cover capacity ranges and
+  // cursor[2 t] is the end of tile t's filled entries (NULL otherwise)
+  const unsigned long long* cursor;
+  int hist_cells;            // table mode: int64 cells of the wide classes' histogram buffer
+                             // (dynamic LDS, set by the launcher; 0: every class walks directly)
+  int pad2;
 };
 
 enum : int { kErrNonFinite = 1, kErrWrap = 2, kErrNegMass = 4, kErrIndexRange = 8 };

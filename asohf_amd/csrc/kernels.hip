@@ -23,44 +23,6 @@ __host__ __device__ constexpr int ktab_off(int irk) { return (2 << irk) - 2 + ir
 constexpr int kKtabLds = ktab_off(kNLevels + 1);
 
 // ---------------------------------------------------------------------------
-// wave / block reductions
-// ---------------------------------------------------------------------------
-template <class T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
-}
-__device__ __forceinline__ unsigned wave_or(unsigned v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-  return v;
-}
-
-// fixed-shape block sum (deterministic for a given blockDim)
-template <class T>
-__device__ T block_sum(T v, T* sh) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  v = wave_sum(v);
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  T r = 0;
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < nw; ++k) r += sh[k];
-    sh[0] = r;
-  }
-  __syncthreads();
-  r = sh[0];
-  __syncthreads();
-  return r;
-}
-
-// ---------------------------------------------------------------------------
 // fills (used instead of hipMemsetAsync so every step can live in a graph)
 // ---------------------------------------------------------------------------
 __global__ void fill_u32_kernel(unsigned* p, unsigned v, long long n) {
@@ -524,9 +486,12 @@ __global__ void plan_write(const unsigned long long* start, int ntiles, int chun
     const long long b = s0 + (long long)c * chunk;
     items[base + k] = Item{t, k << 8, b, min(b + (long long)chunk, s1)};
   }
+  // bit 1: the item holds every wide entry of its tile (the histogram form
+  // is used only then, so its one rounding per cell does not depend on how
+  // the entries fall into items)
   for (int c = 0; c < cw; ++c, ++k) {
     const long long b = s1 + (long long)c * chunk_wide;
-    items[base + k] = Item{t, 1 | (k << 8), b, min(b + (long long)chunk_wide, s2)};
+    items[base + k] = Item{t, 1 | (cw == 1 ? 2 : 0) | (k << 8), b, min(b + (long long)chunk_wide, s2)};
   }
 }
 
@@ -671,9 +636,10 @@ constexpr int kMergeMin = 4;
 // order of its LDS atomics is.
 // Three butterfly steps: lane l takes lane l^1's terms when both stencils
 // start at the same cell (l even), then lane l^2's (bit 1 of l clear), then
-// lane l^4's (bit 2 clear).  Every non-centre TSC term is at most
-// 0.5 * 0.75 * 0.75 of Q(m) < 2^31 (one factor is an outer weight; table mode
-// merges only when the plan found every non-centre product of its row-0
+// lane l^4's (bit 2 clear).  Every non-centre TSC term is below half of
+// Q(m) < 2^31 (one factor is an outer weight: at most 0.5 * 0.75 * 0.75 with
+// exact centres, below 1/2 with the centre rounding the plan admits; table
+// mode merges only when the plan found every non-centre product of its row-0
 // weights at most 0.45), so four of them sum below 2^32 and the third step
 // adds in 64 bits; the centre remainder, negative for a light enough
 // particle, merges as int64.
@@ -781,6 +747,78 @@ __device__ __forceinline__ int centre_local(int c, int org, int ext, int n, bool
   return l;
 }
 
+// The wide entries of one class (kernel level irk, half-width N) of a work
+// item, in the histogram form: the fixed-point masses of the class's members
+// summed at their cells (the reference's IX, JY, KZ) on the tile's box
+// extended by N cells (every member of the item has its cell there: its box
+// meets the tile), exact int64 sums; then the kernel row applied along x, y
+// and z in FP64, each pass in place on the buffer; the result, rounded once
+// per cell, is added to the tile's accumulator.  Cells of the tile outside
+// the grid receive values that are never written out.
+template <int RED, int TX, int TY, int TZ, int HX, int HY>
+__device__ void wide_histogram_form(const DepArgs& D, const Item& it, int irk, int i0, int j0, int k0c,
+                                    const float* K, float scale, double dscale,
+                                    typename AccT<RED>::T* acc, long long* hist) {
+  using Acc = typename AccT<RED>::T;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int nk = 1 << irk;
+  const int bx = TX + 2 * nk, by = TY + 2 * nk, bz = TZ + 2 * nk, nb = bx * by * bz;
+  double* hd = reinterpret_cast<double*>(hist);
+  auto at = [](int a, int b, int c) { return (a + 2) + HX * ((b + 2) + HY * (c + 2)); };
+  for (int c = tid; c < nb; c += nthr) hist[c] = 0;
+  __syncthreads();
+  for (long long e = it.begin + tid; e < it.end; e += nthr) {
+    int ci[3], irk_e;
+    float m;
+    unpack_table(D.entries[e], ci[0], ci[1], ci[2], irk_e, m);
+    if (irk_e != irk) continue;
+    const int hx = ci[0] - i0 + nk, hy = ci[1] - j0 + nk, hz = ci[2] - k0c + nk;
+    if (hx < 0 || hx >= bx || hy < 0 || hy >= by || hz < 0 || hz >= bz) continue;
+    atomicAdd(reinterpret_cast<unsigned long long*>(hist + hx + bx * (hy + by * hz)),
+              (unsigned long long)__float2ll_rn(m * scale));
+  }
+  __syncthreads();
+  for (int c = tid; c < nb; c += nthr) hd[c] = (double)hist[c];
+  __syncthreads();
+  // x: every (y, z) row of the box; out(xo) = sum_d row(xo + N - d) K(d),
+  // written over row(xo), which no later output of the row reads
+  for (int r = tid; r < by * bz; r += nthr) {
+    double* row = hd + r * bx;
+    for (int xo = 0; xo < TX; ++xo) {
+      double s = 0.0;
+      for (int d = -nk; d <= nk; ++d) s += row[xo + nk - d] * (double)K[d];
+      row[xo] = s;
+    }
+  }
+  __syncthreads();
+  // y: every column (x of the tile, z of the box)
+  for (int r = tid; r < TX * bz; r += nthr) {
+    double* col = hd + (r % TX) + bx * by * (r / TX);
+    for (int yo = 0; yo < TY; ++yo) {
+      double s = 0.0;
+      for (int d = -nk; d <= nk; ++d) s += col[bx * (yo + nk - d)] * (double)K[d];
+      col[bx * yo] = s;
+    }
+  }
+  __syncthreads();
+  // z: every (x, y) of the tile, into the accumulator
+  const double inv_scale = 1.0 / dscale;   // a power of two: exact
+  for (int r = tid; r < TX * TY; r += nthr) {
+    const int xo = r % TX, yo = r / TX;
+    const double* col = hd + xo + bx * yo;
+    for (int zo = 0; zo < TZ; ++zo) {
+      double s = 0.0;
+      for (int d = -nk; d <= nk; ++d) s += col[bx * by * (zo + nk - d)] * (double)K[d];
+      if (s != 0.0) {
+        Acc* dst = acc + at(xo, yo, zo);
+        if (RED == 0) *reinterpret_cast<long long*>(dst) += __double2ll_rn(s);
+        else *reinterpret_cast<float*>(dst) += (float)(s * inv_scale);
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <int MODE, int RED, int TX, int TY, int TZ>
 __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
   using Acc = typename AccT<RED>::T;
@@ -796,6 +834,10 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
   // table mode: the used part of the kernel table, row irk (2^(irk+1) + 1
   // values) at ktab_off(irk), centred at ktab_off(irk) + 2^irk
   __shared__ float s_ktab[MODE == 1 ? kKtabLds : 1];
+  __shared__ unsigned s_ccnt[kNLevels + 1];
+  // table mode: the histogram buffer of the wide classes (dynamic LDS,
+  // D.hist_cells int64 cells)
+  extern __shared__ long long s_hist[];
   const int tid = threadIdx.x, nthr = blockDim.x;
   const double dscale = fixed_scale(D.C->mmax_bits);
   const float scale = (float)dscale;
@@ -844,7 +886,7 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
       // loop runs the same number of passes on every lane (ok = the lane holds
       // an entry), which the lane merging of the TSC deposit needs.  An empty
       // item reads nothing: no pass runs.
-      const long long len = it.end - it.begin;
+      const long long len = (D.cursor ? min(it.end, (long long)D.cursor[2 * t]) : it.end) - it.begin;
       auto narrow = [&](const int4 v, const bool ok) {
         int ci[3];
         float wx[3], wy[3], wz[3], m;
@@ -852,19 +894,21 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
           const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
           m = __int_as_float(v.w);
           float* w[3] = {wx, wy, wz};
-          // weights from the offset to the centre cell (RAD(I) on the base
-          // grid, MINIRADX(I) on a patch); the indices are the reference's
+          // weights from the three centres RAD(I-1..I+1) on the base grid,
+          // MINIRADX(I-1..I+1) on a patch; the indices are the reference's
           if (g == 0) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]);
-              tsc_weights_t((p[a] - base_centre(P, a, ci[a], G.n[a])) * G.inv_d[a], w[a]);
+              tsc_weights_f64(p[a], base_centre(P, a, ci[a] - 1, G.n[a]), base_centre(P, a, ci[a], G.n[a]),
+                              base_centre(P, a, ci[a] + 1, G.n[a]), G.inv_dd[a], w[a]);
             }
           } else {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               ci[a] = (int)index_quotient(p[a] - G.org[a], G.d[a], G.inv_d[a]);
-              tsc_weights_t((p[a] - minirad(G.pr[a], ci[a], G.d[a])) * G.inv_d[a], w[a]);
+              tsc_weights_f64(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
+                              minirad(G.pr[a], ci[a] + 1, G.d[a]), G.inv_dd[a], w[a]);
             }
           }
         } else {
@@ -885,7 +929,7 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
         else if (ok)
           deposit27<RED, HX, HY>(acc + corner, m, wx, wy, wz, scale);
       };
-      const long long last = it.end - 1;
+      const long long last = it.begin + len - 1;
       int4 va = make_int4(0, 0, 0, 0), vb = va;
       const long long e0 = it.begin + tid;
       if (len > 0) {
@@ -899,58 +943,96 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
         vb = D.entries[min(e0 + o + 3 * nthr, last)];
       }
     } else if (MODE == 1) {
-      // ---- wide table-mode entries: one wave per entry.  The entry's box
-      // clipped to the tile (bx x by x bz cells) is walked as one flat range,
-      // lane l taking cells l, l + 64, ...; its (x, y, z) counters advance by
-      // constant steps, so the loop has no divisions.  Kernel values come
-      // from the LDS copy of the table; the next entry is loaded while this
-      // one deposits.  Terms are ((m*Kx)*Ky)*Kz as in the reference's loop. ----
-      const int wave = tid >> 6, lane = tid & 63, nwaves = nthr >> 6;
-      long long e = it.begin + wave;
-      int4 nxt = e < it.end ? D.entries[e] : make_int4(0, 0, 0, 0);
-      for (; e < it.end; e += nwaves) {
-        int ci[3], irk;
-        float m;
-        unpack_table(nxt, ci[0], ci[1], ci[2], irk, m);
-        if (e + nwaves < it.end) nxt = D.entries[e + nwaves];
+      // ---- wide table-mode entries (kernel level irk >= 1, half-width
+      // N = 2^irk), one class at a time.  Classes whose halo-extended box
+      // fits the histogram buffer: an NGP histogram of the members' masses
+      // at their cells, then the separable kernel, three 1-D passes (the
+      // histogram form of grids.f:888-951).  Larger classes: the direct walk
+      // of every entry's clipped box. ----
+      // entries per class (wave ballots, one LDS add per wave and class)
+      if (tid <= kNLevels) s_ccnt[tid] = 0u;
+      __syncthreads();
+      {
+        unsigned cnt[kNLevels + 1] = {};
+        for (long long b0 = it.begin; b0 < it.end; b0 += nthr) {
+          const long long e = b0 + tid;
+          const int irk_e = e < it.end ? (D.entries[e].y >> 16) : 0;
+#pragma unroll
+          for (int k = 1; k <= kNLevels; ++k) cnt[k] += (unsigned)__popcll(__ballot(irk_e == k));
+        }
+        if ((tid & 63) == 0)
+          for (int k = 1; k <= kNLevels; ++k)
+            if (cnt[k]) atomicAdd(&s_ccnt[k], cnt[k]);
+      }
+      __syncthreads();
+      for (int irk = 1; irk <= kNLevels; ++irk) {
+        const unsigned ncls = s_ccnt[irk];
+        if (ncls == 0u) continue;
         const int nk = 1 << irk;
-        const float* K = s_ktab + ktab_off(irk) + nk;
-        const int xa = max(max(ci[0] - nk, 1), i0), xb = min(min(ci[0] + nk, G.n[0]), i0 + ex - 1);
-        const int ya = max(max(ci[1] - nk, 1), j0), yb = min(min(ci[1] + nk, G.n[1]), j0 + ey - 1);
-        const int za = max(max(ci[2] - nk, 1), k0c), zb = min(min(ci[2] + nk, G.n[2]), k0c + ez - 1);
-        if (xb < xa || yb < ya || zb < za) continue;
-        const int bx = xb - xa + 1, by = yb - ya + 1, total = bx * by * (zb - za + 1);
-        // q / n for q <= 64 * 16 and n <= 16: (q + 1/2) / n is at least 1/32
-        // away from an integer, far beyond the FP32 error of the product
-        const float rbx = 1.0f / (float)bx, rby = 1.0f / (float)by;
-        auto divn = [](int q, float rn) { return (int)(((float)q + 0.5f) * rn); };
-        const int r0 = divn(lane, rbx);
-        int a = lane - r0 * bx;
-        int c = divn(r0, rby);
-        int b = r0 - c * by;
-        const int dr = divn(64, rbx), da = 64 - dr * bx;
-        const int dc0 = divn(dr, rby), db0 = dr - dc0 * by;
-        const int dc1 = divn(dr + 1, rby), db1 = dr + 1 - dc1 * by;
-        const float ms = RED == 0 ? m * scale : m;
-        const float* Kx = K + (xa - ci[0]);
-        const float* Ky = K + (ya - ci[1]);
-        const float* Kz = K + (za - ci[2]);
-        Acc* base = acc + at(xa - i0, ya - j0, za - k0c);
-        for (int q2 = lane; q2 < total; q2 += 64) {
-          const float term = ((ms * Kx[a]) * Ky[b]) * Kz[c];
-          Acc* dst = base + (a + HX * (b + HY * c));
-          if (RED == 0) lds_add(reinterpret_cast<long long*>(dst), (long long)__float2int_rn(term));
-          else lds_add(reinterpret_cast<float*>(dst), term);
-          a += da;
-          const bool carry = a >= bx;
-          if (carry) a -= bx;
-          b += carry ? db1 : db0;
-          c += carry ? dc1 : dc0;
-          if (b >= by) {
-            b -= by;
-            ++c;
+        const float* K = s_ktab + ktab_off(irk) + nk;   // K[d], d in [-N, N]
+        const int bx = TX + 2 * nk, by = TY + 2 * nk, bz = TZ + 2 * nk;
+        // the histogram form when its buffer fits and its three passes cost
+        // less than the entries' direct adds (an LDS add counted as four
+        // FP64 multiply-adds)
+        const long long w = 2 * nk + 1;
+        const long long passes = w * ((long long)bx * by * TX + (long long)TX * bz * TY + (long long)TX * TY * TZ);
+        if ((it.flags & 2) && bx * by * bz <= D.hist_cells && 4ll * ncls * w * w * w > passes) {
+          wide_histogram_form<RED, TX, TY, TZ, HX, HY>(D, it, irk, i0, j0, k0c, K, scale, dscale, acc, s_hist);
+          continue;
+        }
+        // direct: one wave per entry.  The entry's box clipped to the tile
+        // (bx x by x bz cells) is walked as one flat range, lane l taking
+        // cells l, l + 64, ...; its (x, y, z) counters advance by constant
+        // steps, so the loop has no divisions.  Kernel values come from the
+        // LDS copy of the table; the next entry is loaded while this one
+        // deposits.  Terms are ((m*Kx)*Ky)*Kz as in the reference's loop.
+        const int wave = tid >> 6, lane = tid & 63, nwaves = nthr >> 6;
+        long long e = it.begin + wave;
+        int4 nxt = e < it.end ? D.entries[e] : make_int4(0, 0, 0, 0);
+        for (; e < it.end; e += nwaves) {
+          int ci[3], irk_e;
+          float m;
+          unpack_table(nxt, ci[0], ci[1], ci[2], irk_e, m);
+          if (e + nwaves < it.end) nxt = D.entries[e + nwaves];
+          if (irk_e != irk) continue;
+          const int xa = max(max(ci[0] - nk, 1), i0), xb = min(min(ci[0] + nk, G.n[0]), i0 + ex - 1);
+          const int ya = max(max(ci[1] - nk, 1), j0), yb = min(min(ci[1] + nk, G.n[1]), j0 + ey - 1);
+          const int za = max(max(ci[2] - nk, 1), k0c), zb = min(min(ci[2] + nk, G.n[2]), k0c + ez - 1);
+          if (xb < xa || yb < ya || zb < za) continue;
+          const int cx = xb - xa + 1, cy = yb - ya + 1, total = cx * cy * (zb - za + 1);
+          // q / n for q <= 64 * 16 and n <= 16: (q + 1/2) / n is at least 1/32
+          // away from an integer, far beyond the FP32 error of the product
+          const float rbx = 1.0f / (float)cx, rby = 1.0f / (float)cy;
+          auto divn = [](int q, float rn) { return (int)(((float)q + 0.5f) * rn); };
+          const int r0 = divn(lane, rbx);
+          int a = lane - r0 * cx;
+          int c = divn(r0, rby);
+          int b = r0 - c * cy;
+          const int dr = divn(64, rbx), da = 64 - dr * cx;
+          const int dc0 = divn(dr, rby), db0 = dr - dc0 * cy;
+          const int dc1 = divn(dr + 1, rby), db1 = dr + 1 - dc1 * cy;
+          const float ms = RED == 0 ? m * scale : m;
+          const float* Kx = K + (xa - ci[0]);
+          const float* Ky = K + (ya - ci[1]);
+          const float* Kz = K + (za - ci[2]);
+          Acc* base = acc + at(xa - i0, ya - j0, za - k0c);
+          for (int q2 = lane; q2 < total; q2 += 64) {
+            const float term = ((ms * Kx[a]) * Ky[b]) * Kz[c];
+            Acc* dst = base + (a + HX * (b + HY * c));
+            if (RED == 0) lds_add(reinterpret_cast<long long*>(dst), (long long)__float2int_rn(term));
+            else lds_add(reinterpret_cast<float*>(dst), term);
+            a += da;
+            const bool carry = a >= cx;
+            if (carry) a -= cx;
+            b += carry ? db1 : db0;
+            c += carry ? dc1 : dc0;
+            if (b >= cy) {
+              b -= cy;
+              ++c;
+            }
           }
         }
+        __syncthreads();
       }
     }
     __syncthreads();
@@ -1063,11 +1145,17 @@ __global__ void __launch_bounds__(256) split_reduce(Plan P, DepArgs D) {
 // the device); a kernel, not a memset, so the build can be captured
 __global__ void __launch_bounds__(256) zero_split(DepArgs D, int T, int elem_words) {
   if (D.C->overflow) return;
-  const unsigned long long n = *D.n_split_tiles * (unsigned long long)T * (unsigned long long)elem_words;
+  const unsigned long long ns = *D.n_split_tiles;
+  const unsigned long long n = ns * (unsigned long long)T * (unsigned long long)elem_words;
   unsigned* p = reinterpret_cast<unsigned*>(D.scratch);
   for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < n;
        i += (unsigned long long)gridDim.x * blockDim.x)
     p[i] = 0u;
+  // the split tiles' masses, which split_reduce accumulates (zero bits are 0
+  // as int64 and as FP64)
+  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < ns;
+       i += (unsigned long long)gridDim.x * blockDim.x)
+    D.tile_mass_i[D.split_tiles[i]] = 0;
 }
 
 // per-grid mass in a fixed order; FP64 (int64-exact in deterministic mode)
@@ -1122,6 +1210,24 @@ __global__ void __launch_bounds__(256) cell_keys(Plan P, Parts A, unsigned* key,
   }
 }
 
+// One (particle, grid) term of the adjoint: the nine rows of three gradient
+// values around the centre cell, weighted.  Per row (b, c):
+// sx = sum_a wx_a g, sdx = sum_a wx'_a g; then am += wy wz sx,
+// ax += wy wz sdx, ay += wy' wz sx, az += wy dwz sx.
+struct AdjAcc {
+  float am = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+  __device__ __forceinline__ void row(const float gv[3], const float w[3][3], const float dw[3][3], int b, int c,
+                                      float sc) {
+    const float sx = (w[0][0] * gv[0] + w[0][1] * gv[1] + w[0][2] * gv[2]) * sc;
+    const float sdx = (dw[0][0] * gv[0] + dw[0][1] * gv[1] + dw[0][2] * gv[2]) * sc;
+    const float yz = w[1][b] * w[2][c];
+    am += yz * sx;
+    ax += yz * sdx;
+    ay += dw[1][b] * w[2][c] * sx;
+    az += w[1][b] * dw[2][c] * sx;
+  }
+};
+
 // One thread per particle, in the given order (NULL: index order).  The
 // result of a particle does not depend on the order.
 __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const float* g1, const float* g11,
@@ -1132,19 +1238,20 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
     const long long i = order ? (long long)order[t] : t;
     const float p[3] = {A.x[i], A.y[i], A.z[i]};
     const float m = A.m[i];
-    float am = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+    AdjAcc acc;
+    // the first level's hash slot is loaded while the base grid is handled
+    Probe nxt;
+    if (P.nl >= 1) nxt = probe_start(P, 1, p[0], p[1], p[2]);
     // base grid
     {
       const GridDesc& G = P.grids[0];
-      const float* rad[3] = {P.radx, P.rady, P.radz};
       int ci[3];
       float w[3][3], dw[3][3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         ci[a] = tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]);
-        const float c0 = rad[a][ci[a] - 1], c1 = rad[a][ci[a]], c2 = rad[a][ci[a] + 1];
-        tsc_weights(p[a], c0, c1, c2, G.d[a], w[a]);
-        tsc_dweights(p[a], c0, c1, c2, G.d[a], dw[a]);
+        tsc_weights_dc(p[a], base_centre(P, a, ci[a] - 1, G.n[a]), base_centre(P, a, ci[a], G.n[a]),
+                       base_centre(P, a, ci[a] + 1, G.n[a]), G.inv_d[a], w[a], dw[a]);
       }
       const float sc = 1.0f / (G.den * P.fdm);
       // the three x cells of a row are contiguous unless the row wraps
@@ -1163,49 +1270,41 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
 #pragma unroll
             for (int a = 0; a < 3; ++a) gv[a] = row[wrap1(ci[0] - 1 + a, G.n[0]) - 1];
           }
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const float v = gv[a] * sc;
-            am += v * w[0][a] * w[1][b] * w[2][c];
-            ax += v * dw[0][a] * w[1][b] * w[2][c];
-            ay += v * w[0][a] * dw[1][b] * w[2][c];
-            az += v * w[0][a] * w[1][b] * dw[2][c];
-          }
+          acc.row(gv, w, dw, b, c, sc);
         }
       }
     }
     // the descent stops where the forward enumeration stops (nested boxes)
     for (int l = 1; l <= P.nl; ++l) {
-      const int2 se = bucket_lookup(P, l, p[0], p[1], p[2]);
+      const int2 se = probe_finish(P, nxt);
+      if (l < P.nl) nxt = probe_start(P, l + 1, p[0], p[1], p[2]);
       bool any = false;
       for (int q = 0; q < se.y; ++q) {
-        const BoxRec r = P.hrec[se.x + q];
-        if (!in_box(r, p[0], p[1], p[2])) continue;
+        const BoxRec& G = P.hrec[se.x + q];   // the grid's fields travel in its candidate record
+        if (!in_box(G, p[0], p[1], p[2])) continue;
         any = true;
-        const int g = r.g;
-        const BoxRec& G = r;   // the grid's fields travel in its candidate record
         int ci[3];
         float w[3][3], dw[3][3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           ci[a] = (int)index_quotient(p[a] - G.org[a], G.d[a], G.inv_d[a]);
-          const float c0 = minirad(G.pr[a], ci[a] - 1, G.d[a]), c1 = minirad(G.pr[a], ci[a], G.d[a]),
-                      c2 = minirad(G.pr[a], ci[a] + 1, G.d[a]);
-          tsc_weights(p[a], c0, c1, c2, G.d[a], w[a]);
-          tsc_dweights(p[a], c0, c1, c2, G.d[a], dw[a]);
+          tsc_weights_dc(p[a], minirad(G.pr[a], ci[a] - 1, G.d[a]), minirad(G.pr[a], ci[a], G.d[a]),
+                         minirad(G.pr[a], ci[a] + 1, G.d[a]), G.inv_d[a], w[a], dw[a]);
         }
         const float sc = 1.0f / (G.den * P.fdm);
         const float* gp = g11 + G.off;
         const bool xin = ci[0] >= 2 && ci[0] <= G.n[0] - 1;
+        const long long ld1 = G.ld1, ld2 = G.ld2;
+        const int gn0 = G.n[0], gn1 = G.n[1], gn2 = G.n[2];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const int kz = ci[2] - 1 + c;
-          if (kz < 1 || kz > G.n[2]) continue;
+          if (kz < 1 || kz > gn2) continue;
 #pragma unroll
           for (int b = 0; b < 3; ++b) {
             const int jy = ci[1] - 1 + b;
-            if (jy < 1 || jy > G.n[1]) continue;
-            const float* row = gp + (long long)G.ld1 * (jy - 1) + (long long)G.ld2 * (kz - 1);
+            if (jy < 1 || jy > gn1) continue;
+            const float* row = gp + ld1 * (jy - 1) + ld2 * (kz - 1);
             float gv[3];
             if (xin) {
               load_row3(row + (ci[0] - 2), gv);
@@ -1213,26 +1312,19 @@ __global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const flo
 #pragma unroll
               for (int a = 0; a < 3; ++a) {
                 const int ix = ci[0] - 1 + a;
-                gv[a] = (ix >= 1 && ix <= G.n[0]) ? row[ix - 1] : 0.f;
+                gv[a] = (ix >= 1 && ix <= gn0) ? row[ix - 1] : 0.f;
               }
             }
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const float v = gv[a] * sc;
-              am += v * w[0][a] * w[1][b] * w[2][c];
-              ax += v * dw[0][a] * w[1][b] * w[2][c];
-              ay += v * w[0][a] * dw[1][b] * w[2][c];
-              az += v * w[0][a] * w[1][b] * dw[2][c];
-            }
+            acc.row(gv, w, dw, b, c, sc);
           }
         }
       }
       if (!any && P.nested) break;
     }
-    dm[i] = am;
-    dxo[i] = m * ax;
-    dyo[i] = m * ay;
-    dzo[i] = m * az;
+    dm[i] = acc.am;
+    dxo[i] = m * acc.ax;
+    dyo[i] = m * acc.ay;
+    dzo[i] = m * acc.az;
   }
 }
 
@@ -1515,9 +1607,39 @@ hipError_t launch_gate(const unsigned long long* tot_entries, const unsigned lon
   return hipGetLastError();
 }
 
+// Table mode: the histogram buffer of the wide classes takes the LDS left
+// beside the kernel's static arrays, sized for the largest class of the
+// geometry whose extended box fits there (smaller classes use a part of it).
 template <int MODE, int RED, int TX, int TY, int TZ>
 static hipError_t dep(const Plan& P, const DepArgs& D, int grid, int block, hipStream_t s) {
-  deposit_kernel<MODE, RED, TX, TY, TZ><<<grid, block, 0, s>>>(P, D);
+  DepArgs d2 = D;
+  d2.hist_cells = 0;
+  size_t dyn = 0;
+  if (MODE == 1) {
+    static long long avail = -1;   // int64 cells of LDS beside the static arrays (per instantiation)
+    if (avail < 0) {
+      int dev = 0, maxl = 0;
+      hipFuncAttributes fa;
+      hipError_t e = hipGetDevice(&dev);
+      if (e == hipSuccess) e = hipDeviceGetAttribute(&maxl, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+      if (e == hipSuccess)
+        e = hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(&deposit_kernel<MODE, RED, TX, TY, TZ>));
+      if (e != hipSuccess) return e;
+      avail = std::max<long long>(((long long)maxl - (long long)fa.sharedSizeBytes - 256) / 8, 0);
+      if (avail > 0) {
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&deposit_kernel<MODE, RED, TX, TY, TZ>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(avail * 8));
+        if (e != hipSuccess) return e;
+      }
+    }
+    for (int irk = 1; irk <= P.nl; ++irk) {
+      const long long nk = 1ll << irk;
+      const long long nb = (TX + 2 * nk) * (TY + 2 * nk) * (TZ + 2 * nk);
+      if (nb <= avail) d2.hist_cells = (int)nb;
+    }
+    dyn = sizeof(long long) * (size_t)d2.hist_cells;
+  }
+  deposit_kernel<MODE, RED, TX, TY, TZ><<<grid, block, dyn, s>>>(P, d2);
   return hipGetLastError();
 }
 

@@ -36,6 +36,18 @@ hipError_t launch_deposit(const Plan& P, const DepArgs& D, int reduction, int gr
 hipError_t launch_zero_split(const Plan& P, const DepArgs& D, int reduction, int cus, hipStream_t s);
 hipError_t launch_split_reduce(const Plan& P, const DepArgs& D, int reduction, int cus, hipStream_t s);
 hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, double* grid_mass, hipStream_t s);
+// TSC binning along the hierarchy (hier.hip)
+hipError_t launch_cursor_reset(const unsigned long long* start, unsigned long long* cursor, long long nkeys,
+                               int cus, hipStream_t s);
+hipError_t launch_count_to_capacity(unsigned* cnt, long long nkeys, int cus, hipStream_t s);
+hipError_t launch_bin_base_tsc(const Plan& P, const Parts& A, unsigned long long* cursor,
+                               const unsigned long long* start, int4* entries, Counters* C,
+                               const BlockTables& BT, hipStream_t s);
+hipError_t launch_emit_tsc(const Plan& P, const Item* items, long long item_lo, long long item_hi,
+                           unsigned long long* cursor, const unsigned long long* start, int4* entries,
+                           unsigned* item_pairs, Counters* C, int cus, hipStream_t s);
+hipError_t launch_hier_stats(const unsigned* item_pairs, long long n_items, const unsigned long long* start,
+                             const unsigned long long* cursor, long long nkeys, Counters* C, hipStream_t s);
 hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
                           float* dx, float* dy, float* dz, const unsigned* order, int cus, hipStream_t s);
 hipError_t launch_cell_keys(const Plan& P, const Parts& A, unsigned* key, unsigned* idx, int cus, hipStream_t s);

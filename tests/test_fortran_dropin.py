@@ -94,8 +94,11 @@ def test_fortran_density_matches_reference(name):
         assert r.returncode == 0, r.stdout + r.stderr
         raw = np.fromfile(fout, np.float32)
     nc = g.ncells_base
-    u1 = raw[:nc].reshape(g.nz, g.ny, g.nx)
-    u11 = raw[nc:-1]
+    # DENSITY's output precedes RENORM_DENSITY, which the reference pipeline
+    # runs afterwards (asohf.f:527-528): divide by FDM as it does (FP32)
+    fdm = np.float32(g.fdm)
+    u1 = raw[:nc].reshape(g.nz, g.ny, g.nx) / fdm
+    u11 = raw[nc:-1] / fdm
     changed = raw[-1:].view(np.int32)[0]
     assert changed == 0                       # padding keeps INIT_GRIDVARS' -1
     assert "At level" in r.stdout             # the reference's per-level report

@@ -7,12 +7,16 @@ only if every non-centre term stays below 2^32 / 4.  A term is
 weight products:
 
   * TSC: the weights of a particle at offset t (in cells) from its centre cell
-    are 0.5 (0.5 - t)^2, 0.75 - t^2, 0.5 (0.5 + t)^2 (grids.f:1194-1213 in
-    the form the kernels evaluate), and |t| <= 1/2 up to rounding for a
-    particle inside the grid or inside a patch's BUF box (the index is the
-    cell that contains it); the base grid accepts positions up to 2^-10
-    cells past the box, so |t| <= 1/2 + 2^-10 there.  A non-centre product has at least one outer
-    weight, so it is at most 0.5 * 0.75 * 0.75 = 0.28125.
+    are 0.5 (0.5 - t)^2, 0.75 - t^2, 0.5 (0.5 + t)^2 (grids.f:1194-1213),
+    and |t| <= 1/2 up to rounding for a particle inside the grid or inside a
+    patch's BUF box (the index is the cell that contains it); the base grid
+    accepts positions up to 2^-10 cells past the box, so |t| <= 1/2 + 2^-10
+    there.  A non-centre product has at least one outer weight, so it is at
+    most 0.5 * 0.75 * 0.75 = 0.28125.  The kernels evaluate the weights, as
+    the reference does, from the three FP32 centres of the stencil, each
+    rounded on its own; the plan refuses grids where their spacing may miss
+    the cell size by more than a quarter of a cell, and within that every
+    weight stays non-negative and every non-centre product below 1/2.
   * Table mode: narrow entries use row 0 of COMPUTE_KERNELS' table
     (grids.f:997-1067); the plan merges only when every non-centre product of
     that row is at most 0.45.
@@ -70,6 +74,16 @@ def test_tsc_weight_products_fit_the_merge():
     assert np.all(ww >= 0)
     bound = max(bound, max_non_centre_product(ww.max(axis=1)))
     assert bound <= 0.2822, bound
+    # the three centres rounded on their own: each distance off by up to a
+    # quarter of a cell (the plan's limit)
+    rng = np.random.default_rng(3)
+    tt = rng.uniform(-0.5 - 2.0 ** -10, 0.5 + 2.0 ** -10, 200000)
+    dd = rng.uniform(-0.25, 0.25, (3, 200000))
+    b = np.stack([np.abs(1 + tt + dd[0]), np.abs(tt + dd[1]), np.abs(1 - tt + dd[2])]).astype(f32)
+    wc = np.stack([f32(0.5) * (f32(1.5) - b[0]) ** 2, f32(0.75) - b[1] * b[1], f32(0.5) * (f32(1.5) - b[2]) ** 2])
+    assert np.all(wc >= 0)
+    # below 1/2: four terms of a mass below 2^31 still sum below 2^32
+    assert max_non_centre_product(wc.max(axis=1)) < 0.5
     # four terms of a mass below 2^31 fit 32 bits; eight do not (hence the
     # third step's 64-bit add)
     q_max = np.floor(bound * 2.0 ** 31) + 1
