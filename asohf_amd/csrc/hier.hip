@@ -268,12 +268,18 @@ __global__ void __launch_bounds__(kEmThreads) emit_tsc(Plan P, const Item* items
             if (!in_box(R, p[0], p[1], p[2])) continue;
             ++pairs;
             int lo[3], hi[3];
+            bool empty = false;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               const int ci = (int)index_quotient(p[a] - R.org[a], R.d[a], R.inv_d[a]);
-              lo[a] = (max(ci - 1, 1) - 1) >> P.tshift[a];
-              hi[a] = (min(ci + 1, R.n[a]) - 1) >> P.tshift[a];
+              const int c0 = max(ci - 1, 1), c1 = min(ci + 1, R.n[a]);
+              // a member whose quotient rounds onto the far face of the box
+              // (ci = N + 2) has no stencil cell in the patch
+              empty = empty || c1 < c0;
+              lo[a] = (c0 - 1) >> P.tshift[a];
+              hi[a] = (c1 - 1) >> P.tshift[a];
             }
+            if (empty) continue;
             for (int cz = lo[2]; cz <= hi[2]; ++cz)
               for (int cy = lo[1]; cy <= hi[1]; ++cy)
                 for (int cx = lo[0]; cx <= hi[0]; ++cx) {
