@@ -24,7 +24,7 @@ PATH_BATCHED, PATH_PLAIN = 0, 1
 EXPORTS = ("asohf_hip_create", "asohf_hip_destroy", "asohf_hip_last_error", "asohf_hip_density",
            "asohf_hip_density_host", "asohf_hip_density_adjoint", "asohf_hip_membership",
            "asohf_hip_get_tune", "asohf_hip_set_tune", "asohf_hip_autotune", "asohf_hip_get_stats",
-           "asohf_hip_release_workspaces", "asohf_hip_species", "asohf_hip_peak_candidates",
+           "asohf_hip_release_workspaces", "asohf_hip_species", "asohf_hip_species_noalloc", "asohf_hip_peak_candidates",
            "asohf_hip_refinement_masks", "asohf_hip_overlap_mask", "asohf_hip_write_density_dump",
            "asohf_hip_create_mesh", "asohf_hip_version")
 
@@ -113,6 +113,7 @@ def load():
         L.asohf_hip_release_workspaces.argtypes = [vp]
         L.asohf_hip_species.argtypes = [vp, _ip, C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int64,
                                         vp, vp, vp, vp, vp, vp, _ip, vp]
+        L.asohf_hip_species_noalloc.argtypes = L.asohf_hip_species.argtypes
         L.asohf_hip_peak_candidates.argtypes = [vp, G, C.c_int32, vp, vp, vp, C.c_float, vp, vp, vp, vp, vp,
                                                 C.c_int64, C.POINTER(C.c_int64), O]
         L.asohf_hip_refinement_masks.argtypes = [vp, G, _ip, _ip, _ip, _ip, vp, vp, O]

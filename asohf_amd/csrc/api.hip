@@ -186,6 +186,7 @@ struct asohf_ctx {
   DBuf flags, idx, pool, pool_off, pool_nk, blk_pairs, blk_mmax, nsplit, item_off, split_off;
   DBuf elist, eoff, ecount;
   DBuf ipairs;                             // members found per work item (TSC hierarchy binning)
+  void* noalloc = nullptr;                 // this device's scratch array of order.hip (device address)
   DBuf sp_k[4], sp_dens, sp_cnt, sp_tmp;   // species partition
   DBuf pk_r[2], pk_k[2], pk_tmp, pk_cnt, pk_geo;   // peak candidates
   DBuf mk_geo, mk_err, vg_cnt, vg_start, vg_pairs;  // grid masks
@@ -1325,6 +1326,12 @@ asohf_status asohf_hip_create(asohf_ctx** out, int32_t device) {
     asohf_hip_destroy(ctx);
     return ASOHF_ERR_OOM;
   }
+  // the device address of this GPU's no-allocation scratch array (taken once
+  // per device, with the device current)
+  if (noalloc_scratch_init(ctx->device, &ctx->noalloc) != hipSuccess) {
+    asohf_hip_destroy(ctx);
+    return ASOHF_ERR_HIP;
+  }
   *out = ctx;
   return ASOHF_OK;
 }
@@ -1628,6 +1635,43 @@ asohf_status asohf_hip_species(asohf_ctx* ctx, const int32_t nmax[3], const floa
   unsigned cnt[kNEsp];
   HIPCHK(hipMemcpyAsync(cnt, ctx->sp_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, s), "reading the counts");
   HIPCHK(hipStreamSynchronize(s), "species partition");
+  for (int k = 0; k < kNEsp; ++k) npart_esp[k] = (int32_t)cnt[k];
+  return ASOHF_OK;
+}
+
+asohf_status asohf_hip_species_noalloc(asohf_ctx* ctx, const int32_t nmax[3], const float d[3], float rodo,
+                                       float re0, int64_t n, const float* rx, const float* ry, const float* rz,
+                                       const float* m, int32_t* species, int64_t* order, int32_t* npart_esp,
+                                       void* stream) {
+  if (!ctx) return ASOHF_ERR_INVALID;
+  ctx->err.clear();
+  if (!nmax || !d || !npart_esp || n < 0 || (n > 0 && (!rx || !ry || !rz || !m || !order)))
+    return fail(ctx, ASOHF_ERR_INVALID, "NULL argument");
+  for (int a = 0; a < 3; ++a) {
+    if (nmax[a] < 1 || nmax[a] > 4096) return fail(ctx, ASOHF_ERR_INVALID, "NMAX outside 1..4096");
+    if (!(d[a] > 0.f) || !std::isfinite(d[a])) return fail(ctx, ASOHF_ERR_INVALID, "cell size must be > 0");
+  }
+  const long long nc = (long long)nmax[0] * nmax[1] * nmax[2];
+  if (n > kNoAllocMaxParticles)
+    return fail(ctx, ASOHF_ERR_CAPACITY, "asohf_hip_species_noalloc takes at most 2^20 particles");
+  if (nc > kNoAllocMaxCells)
+    return fail(ctx, ASOHF_ERR_CAPACITY, "asohf_hip_species_noalloc takes at most 2^21 grid cells");
+  if (!ctx->noalloc) return fail(ctx, ASOHF_ERR_HIP, "no scratch array address for this device");
+  HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t s = (hipStream_t)stream;
+  const float bas = (((d[0] * d[1]) * d[2]) * rodo) * powi_f(re0, 3);   // reader.f:203
+  unsigned* counts = nullptr;
+  size_t need = 0;
+  HIPCHK(launch_species_noalloc(ctx->noalloc, nmax, d, bas, n, rx, ry, rz, m, species, (long long*)order,
+                                ctx->cus, s, &counts, &need),
+         "species kernels");
+  if (need > noalloc_temp_bytes())
+    return fail(ctx, ASOHF_ERR_CAPACITY, "the radix sort's temporary storage exceeds the scratch array");
+  // the counts come back through the context's pinned buffer
+  HIPCHK(hipMemcpyAsync(ctx->hscalar, counts, sizeof(unsigned) * kNEsp, hipMemcpyDeviceToHost, s),
+         "reading the counts");
+  HIPCHK(hipStreamSynchronize(s), "species partition");
+  const unsigned* cnt = reinterpret_cast<const unsigned*>(ctx->hscalar);
   for (int k = 0; k < kNEsp; ++k) npart_esp[k] = (int32_t)cnt[k];
   return ASOHF_OK;
 }

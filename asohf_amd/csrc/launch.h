@@ -56,6 +56,16 @@ hipError_t launch_species(const int nmax[3], const float d[3], float bas, long l
                           const float* y, const float* z, const float* m, int* species, long long* order,
                           unsigned* counts, float* dens, unsigned* k0, unsigned* k1, unsigned* k2, unsigned* k3,
                           void* temp, size_t temp_bytes, int cus, hipStream_t s);
+// the species partition on the file-scope scratch array of order.hip
+constexpr long long kNoAllocMaxParticles = 1ll << 20;
+constexpr long long kNoAllocMaxCells = 1ll << 21;
+constexpr size_t kNoAllocBytes = 64ull << 20;
+hipError_t noalloc_scratch_init(int device, void** out);
+hipError_t launch_species_noalloc(void* base, const int nmax[3], const float d[3], float bas, long long n,
+                                  const float* x, const float* y, const float* z, const float* m, int* species,
+                                  long long* order, int cus, hipStream_t s, unsigned** counts_out,
+                                  size_t* temp_need);
+size_t noalloc_temp_bytes();
 hipError_t sort_cell_pairs(void* temp, size_t& temp_bytes, const unsigned* key_in, unsigned* key_out,
                            const unsigned* idx_in, unsigned* idx_out, unsigned long long n, int end_bit,
                            hipStream_t s);

@@ -3,6 +3,7 @@
 // translation unit (the library headers are large).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "launch.h"
@@ -267,5 +268,62 @@ hipError_t launch_species(const int nmax[3], const float d[3], float rodo_re0_ba
   widen_order<<<grid, 256, 0, s>>>(k3, n, order);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// The species partition without allocation (asohf_hip_species_noalloc): its
+// scratch (sort keys and indices, the NGP grid, the class counts, the radix
+// sort's temporary storage) is carved from `scratch`, a file-scope device
+// array that every GPU the library runs on holds until the process ends.
+// In host code `scratch` names a host-side stand-in: its device address is
+// taken once per device with hipGetSymbolAddress and only that address is
+// handed to kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocDevices] = {};
+constexpr size_t kNaKeys = sizeof(unsigned) * (size_t)kNoAllocMaxParticles;   // one key/index array
+constexpr size_t kNaDens = sizeof(float) * (size_t)kNoAllocMaxCells;
+constexpr size_t kNaCounts = 256;
+constexpr size_t kNaTemp = kNoAllocBytes - 4 * kNaKeys - kNaDens - kNaCounts;
+static_assert(4 * kNaKeys + kNaDens + kNaCounts < kNoAllocBytes, "the carve-out fits the array");
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device, void** out) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (device < 0 || device >= kNoAllocDevices) return hipErrorInvalidDevice;
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_noalloc_ptr[device] = p;
+  }
+  *out = g_noalloc_ptr[device];
+  return hipSuccess;
+}
+
+hipError_t launch_species_noalloc(void* base, const int nmax[3], const float d[3], float bas, long long n,
+                                  const float* x, const float* y, const float* z, const float* m, int* species,
+                                  long long* order, int cus, hipStream_t s, unsigned** counts_out,
+                                  size_t* temp_need) {
+  unsigned char* b = static_cast<unsigned char*>(base);
+  unsigned* k[4];
+  for (int i = 0; i < 4; ++i) k[i] = reinterpret_cast<unsigned*>(b + i * kNaKeys);
+  float* dens = reinterpret_cast<float*>(b + 4 * kNaKeys);
+  unsigned* counts = reinterpret_cast<unsigned*>(b + 4 * kNaKeys + kNaDens);
+  void* temp = b + 4 * kNaKeys + kNaDens + kNaCounts;
+  *counts_out = counts;
+  size_t tb = 0;
+  hipError_t e = species_sort_bytes((unsigned long long)(n > 0 ? n : 1), tb);
+  if (e != hipSuccess) return e;
+  *temp_need = tb;
+  if (tb > kNaTemp) return hipSuccess;   // the caller reports the capacity error
+  return launch_species(nmax, d, bas, n, x, y, z, m, species, order, counts, dens, k[0], k[1], k[2], k[3], temp,
+                        kNaTemp, cus, s);
+}
+
+size_t noalloc_temp_bytes() { return kNaTemp; }
 
 }  // namespace asohf

@@ -185,6 +185,20 @@ asohf_status asohf_hip_species(asohf_ctx *ctx, const int32_t nmax[3], const floa
                                const float *m, int32_t *species, int64_t *order, int32_t *npart_esp,
                                void *stream);
 
+/* asohf_hip_species without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as asohf_hip_species, for at most 2^20 particles and
+ * nmax[0]*nmax[1]*nmax[2] <= 2^21 cells (ASOHF_ERR_CAPACITY beyond).  Its
+ * scratch is a 64 MiB array in device memory that every GPU the library runs
+ * on holds until the process ends; every call on a GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other threads, other contexts,
+ * other streams) race.  Synchronises before returning. */
+asohf_status asohf_hip_species_noalloc(asohf_ctx *ctx, const int32_t nmax[3], const float d[3], float rodo,
+                                       float re0, int64_t n, const float *rx, const float *ry, const float *rz,
+                                       const float *m, int32_t *species, int64_t *order, int32_t *npart_esp,
+                                       void *stream);
+
 /* Peak candidates of the grid halo finder: the cells HALOFIND_GRID starts
  * its haloes from (haloes_grids.f:316-404 on the base grid, 700-790 on AMR
  * level IR), in the order SORT_CELLS / SORT_CELLS_AMR (haloes_grids.f:
