@@ -1076,7 +1076,7 @@ static asohf_status enqueue_hier(asohf_ctx* ctx, const PipeArgs& a, hipStream_t 
   }
   for (int l = 0; l < a.nl; ++l) {
     if (!(a.emit_levels & (1u << l))) continue;
-    HIPCHK(launch_emit_tsc(a.P, a.D.items, a.lvl_items[l], a.lvl_items[l + 1], a.cursor, a.start,
+    HIPCHK(launch_emit_tsc(a.P, a.D.items, a.lvl_items[l], a.lvl_items[l + 1], l, a.cursor, a.start,
                            const_cast<int4*>(a.D.entries), a.item_pairs, a.C, a.cus, s),
            "emission pass");
   }

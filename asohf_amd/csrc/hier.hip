@@ -36,14 +36,17 @@ namespace {
 
 constexpr int kHbThreads = 256;
 constexpr int kHbPer = 2;                        // particles per thread
-constexpr int kHbChunk = kHbThreads * kHbPer;    // particles per bin_base workgroup
-constexpr int kHbCap = 1024;                     // LDS key slots (power of two)
-constexpr int kHbList = 8 * kHbChunk;            // at most 8 base tiles per particle: never full
+constexpr int kHbChunk = kHbThreads * kHbPer;    // particles per bin_base workgroup (< 2^16)
+constexpr int kHbCap = 1024;                     // LDS key slots (power of two, < 2^16)
+constexpr int kHbWList = 384;                    // list words per wave
 constexpr int kHbProbe = 32;
 constexpr int kEmThreads = 256;
-constexpr int kEmRound = 1024;                   // entries staged per round of emit_tsc
-constexpr int kEmCap = 1024;
-constexpr int kEmList = 4096;
+constexpr int kEmPer = 2;                        // entries per thread and round
+constexpr int kEmRound = kEmPer * kEmThreads;    // entries per round (< 2^16: a list word's position)
+constexpr int kEmCap = 512;                      // LDS key slots of an item (< 2^16)
+constexpr int kEmWList = 384;                    // list words per wave and round
+constexpr unsigned long long kEmSkip = ~0ull;    // list word of an emission written directly
+static_assert(kEmRound <= 65536 && kEmCap <= 65536, "a list word holds (entry, slot, position)");
 constexpr int kEmTargets = 32;                   // target records staged in LDS at once
 
 __device__ __forceinline__ unsigned slot_hash(unsigned k, unsigned mask) {
@@ -70,6 +73,25 @@ __device__ __forceinline__ int slot_insert(int* keys, int k) {
     h = (h + 1) & (CAP - 1);
   }
   return -1;
+}
+
+// c / e for 0 <= c <= 26 and e in {1, 2, 3} (the tiles of a 3-cell stencil
+// per axis), by a multiply and a shift: c * 86 / 256 lies within 0.07 below
+// c / 3 there, and c / 3 is never within 0.07 above an integer
+__device__ __forceinline__ int div_small(int c, int e) {
+  return (c * (e == 1 ? 256 : (e == 2 ? 128 : 86))) >> 8;
+}
+
+// The cell that decides a member's home tile: INT((x - origin) / d) + 1
+// clamped (the base grid) or INT((x - origin) / d) clamped (a patch) with
+// the quotient taken as a product with the reciprocal.  It is within one cell
+// of the reference's index, so it lies in the member's stencil and its tile
+// holds one of the member's entries; every entry of the member computes the
+// same value, so exactly one of them is home.
+__device__ __forceinline__ int home_cell(float x, float org, float inv_d, bool base, int n) {
+  float t = (x - org) * inv_d;
+  t = fminf(fmaxf(t, -1.0e9f), 1.0e9f);
+  return clampi((int)t + (base ? 1 : 0), 1, n);
 }
 
 __device__ __forceinline__ void set_overflow(Counters* C) {
@@ -106,20 +128,102 @@ __global__ void __launch_bounds__(256) count_to_capacity(unsigned* cnt, long lon
   }
 }
 
+constexpr int kGroups = 4;   // keys of a wave grouped with ballots before lanes add one by one
+
+// The position of one emission per lane that wants one (key = destination
+// tile key) in its key's range of the workgroup's round: the first kGroups
+// keys of the wave are grouped with ballots (no memory operation), the
+// group's first lane finding the key's table slot and adding the group's size
+// to the key's count, whose old value plus the lane's rank in the group is
+// the lane's position; lanes left over look up their key and add 1 each.
+// h = -1 when the table is full (the emission is then not counted).  Every
+// lane of the wave must be active.
+template <int CAP>
+__device__ __forceinline__ void count_round(int* keys, unsigned* vals, bool want, int key, int& h, unsigned& at) {
+  const int lane = (int)__lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  h = -1;
+  at = 0;
+  unsigned long long pend = __ballot(want);
+  for (int gi = 0; gi < kGroups && pend; ++gi) {
+    const int l0 = __ffsll((long long)pend) - 1;
+    const int k0 = __builtin_amdgcn_readlane(key, l0);
+    const bool mine = want && key == k0;
+    const unsigned long long mm = __ballot(mine);
+    int hh = 0;
+    unsigned old = 0;
+    if (lane == l0) {
+      hh = slot_insert<CAP>(keys, k0);
+      if (hh >= 0) old = atomicAdd(&vals[hh], (unsigned)__popcll(mm));
+    }
+    hh = __builtin_amdgcn_readlane(hh, l0);
+    old = (unsigned)__builtin_amdgcn_readlane((int)old, l0);
+    if (mine) {
+      h = hh;
+      at = old + (unsigned)__popcll(mm & below);
+    }
+    pend &= ~mm;
+  }
+  if ((pend >> lane) & 1ull) {
+    h = slot_insert<CAP>(keys, key);
+    if (h >= 0) at = atomicAdd(&vals[h], 1u);
+  }
+}
+
+// exclusive prefix over the wave's lanes of v (0 <= v < 2^BITS) and the total
+template <int BITS>
+__device__ __forceinline__ int wave_prefix(int v, int& total) {
+  const unsigned long long below = (1ull << __lane_id()) - 1ull;
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) {
+    const unsigned long long m = __ballot((v >> b) & 1);
+    off += __popcll(m & below) << b;
+    tot += __popcll(m) << b;
+  }
+  total = tot;
+  return off;
+}
+__device__ __forceinline__ int wave_prefix4(int v, int& total) { return wave_prefix<4>(v, total); }
+__device__ __forceinline__ int wave_prefix5(int v, int& total) { return wave_prefix<5>(v, total); }
+
+// one reservation per tile key of the round: base[k] = the key's first slot
+template <int CAP, int NTHR>
+__device__ __forceinline__ void reserve_round(const int* keys, unsigned* vals, unsigned long long* base,
+                                              unsigned long long* cursor, const unsigned long long* start,
+                                              Counters* C) {
+  for (int k = threadIdx.x; k < CAP; k += NTHR) {
+    const int key = keys[k];
+    const unsigned v = vals[k];
+    if (key >= 0 && v) {
+      const unsigned long long b0 = atomicAdd(&cursor[key], (unsigned long long)v);
+      base[k] = b0;
+      if (b0 + v > start[key + 1]) set_overflow(C);
+    }
+    vals[k] = 0u;
+  }
+}
+
+// Base-grid binning: kHbPer particles per thread.  Each valid particle
+// (finite, mass >= 0, within the TSC range of the base grid) emits one entry
+// per base tile its wrapped stencil touches.  Every wave lists its emissions
+// (tile key, particle) in its own LDS list, then counts the list 64 at a time
+// (a position in the key's range of the workgroup, written back over the
+// list word); one global add per key reserves the workgroup's ranges and
+// every wave writes its listed entries, reading each particle again.
+// Per-workgroup partials: base members and the largest mass.
 __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsigned long long* cursor,
                                                            const unsigned long long* start, int4* entries,
                                                            Counters* C, BlockTables BT) {
   __shared__ int keys[kHbCap];
   __shared__ unsigned vals[kHbCap];
   __shared__ unsigned long long base[kHbCap];
-  __shared__ unsigned list[kHbList];
-  __shared__ int4 parts[kHbChunk];
-  __shared__ int s_n;
+  __shared__ unsigned long long wlist_all[(kHbThreads / 64) * kHbWList];
   __shared__ unsigned long long s_valid[kHbThreads / 64];
   __shared__ float s_mmax[kHbThreads / 64];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = (int)__lane_id();
+  unsigned long long* wl = wlist_all + (tid >> 6) * kHbWList;
   for (int k = tid; k < kHbCap; k += kHbThreads) { keys[k] = -1; vals[k] = 0u; }
-  if (tid == 0) s_n = 0;
   __syncthreads();
   const GridDesc& B = P.grids[0];
   const int bn[3] = {B.n[0], B.n[1], B.n[2]};
@@ -127,64 +231,76 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
   const float bid[3] = {B.inv_d[0], B.inv_d[1], B.inv_d[2]};
   const int bnt0 = B.nt[0], bnt1 = B.nt[1];
   const long long c0 = (long long)blockIdx.x * kHbChunk;
+  auto particle = [&](long long i) {
+    return make_int4(__float_as_int(A.x[i]), __float_as_int(A.y[i]), __float_as_int(A.z[i]), __float_as_int(A.m[i]));
+  };
   unsigned err = 0;
   unsigned long long valid = 0;
   float mmax = 0.f;
+  int nw = 0;   // the wave's list length (uniform)
+#pragma unroll
   for (int r = 0; r < kHbPer; ++r) {
-    const int li = r * kHbThreads + tid;
-    const long long i = c0 + li;
-    if (i >= A.n) break;
-    const float x = A.x[i], y = A.y[i], z = A.z[i], m = A.m[i];
-    const int4 e = make_int4(__float_as_int(x), __float_as_int(y), __float_as_int(z), __float_as_int(m));
-    parts[li] = e;
-    if (!(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) { err |= kErrNonFinite; continue; }
-    if (m < 0.f) { err |= kErrNegMass; continue; }
-    if (!(tsc_base_in_range(x, P.xl, bid[0], bn[0]) && tsc_base_in_range(y, P.xl, bid[1], bn[1]) &&
-          tsc_base_in_range(z, P.xl, bid[2], bn[2]))) {
+    const int k = r * kHbThreads + tid;
+    bool ok = c0 + k < A.n;
+    float x = 0.f, y = 0.f, z = 0.f, m = 0.f;
+    if (ok) { x = A.x[c0 + k]; y = A.y[c0 + k]; z = A.z[c0 + k]; m = A.m[c0 + k]; }
+    if (ok && !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) { err |= kErrNonFinite; ok = false; }
+    if (ok && m < 0.f) { err |= kErrNegMass; ok = false; }
+    if (ok && !(tsc_base_in_range(x, P.xl, bid[0], bn[0]) && tsc_base_in_range(y, P.xl, bid[1], bn[1]) &&
+                tsc_base_in_range(z, P.xl, bid[2], bn[2]))) {
       err |= kErrWrap;
-      continue;
+      ok = false;
     }
-    ++valid;
-    mmax = fmaxf(mmax, m);
-    const int ix = tsc_base_index(x, P.xl, bd[0], bid[0], bn[0]);
-    const int jy = tsc_base_index(y, P.xl, bd[1], bid[1], bn[1]);
-    const int kz = tsc_base_index(z, P.xl, bd[2], bid[2], bn[2]);
-    int tx[3], ty[3], tz[3];
-    const int nx_ = base_axis_tiles(ix, bn[0], P.tshift[0], tx);
-    const int ny_ = base_axis_tiles(jy, bn[1], P.tshift[1], ty);
-    const int nz_ = base_axis_tiles(kz, bn[2], P.tshift[2], tz);
-    for (int c = 0; c < nz_; ++c)
-      for (int b = 0; b < ny_; ++b)
-        for (int a = 0; a < nx_; ++a) {
-          const int key = 2 * ((tz[c] * bnt1 + ty[b]) * bnt0 + tx[a]);
-          const int h = slot_insert<kHbCap>(keys, key);
-          if (h >= 0) {
-            atomicAdd(&vals[h], 1u);
-            list[atomicAdd(&s_n, 1)] = ((unsigned)h << 9) | (unsigned)li;   // at most 8 per particle
-          } else {
-            emit_direct(cursor, start, entries, C, key, e);
-          }
-        }
+    int tx[3] = {0, 0, 0}, ty[3] = {0, 0, 0}, tz[3] = {0, 0, 0};
+    int nt = 0, nx_ = 1, ny_ = 1;
+    if (ok) {
+      ++valid;
+      mmax = fmaxf(mmax, m);
+      nx_ = base_axis_tiles(tsc_base_index(x, P.xl, bd[0], bid[0], bn[0]), bn[0], P.tshift[0], tx);
+      ny_ = base_axis_tiles(tsc_base_index(y, P.xl, bd[1], bid[1], bn[1]), bn[1], P.tshift[1], ty);
+      const int nz_ = base_axis_tiles(tsc_base_index(z, P.xl, bd[2], bid[2], bn[2]), bn[2], P.tshift[2], tz);
+      nt = nx_ * ny_ * nz_;
+    }
+    int tot;
+    const int off = nw + wave_prefix5(nt, tot);
+    for (int c = 0; c < nt; ++c) {
+      const int q = div_small(c, nx_), qz = div_small(q, ny_);
+      const int a = c - q * nx_, b = q - qz * ny_;
+      const int ta = a == 0 ? tx[0] : (a == 1 ? tx[1] : tx[2]);
+      const int tb = b == 0 ? ty[0] : (b == 1 ? ty[1] : ty[2]);
+      const int tcz = qz == 0 ? tz[0] : (qz == 1 ? tz[1] : tz[2]);
+      const int key = 2 * ((tcz * bnt1 + tb) * bnt0 + ta);
+      if (off + c < kHbWList) wl[off + c] = (unsigned long long)(unsigned)key | ((unsigned long long)k << 32);
+      else emit_direct(cursor, start, entries, C, key, particle(c0 + k));
+    }
+    nw = min(nw + tot, kHbWList);
+  }
+  // count the list: a position in the key's range of the workgroup
+  for (int b0 = 0; b0 < nw; b0 += 64) {
+    const int idx = b0 + lane;
+    const bool want = idx < nw;
+    const unsigned long long w = want ? wl[idx] : 0ull;
+    int h;
+    unsigned at;
+    count_round<kHbCap>(keys, vals, want, (int)(unsigned)w, h, at);
+    if (want) {
+      if (h >= 0) {
+        wl[idx] = (w & 0xffffffff00000000ull) | ((unsigned long long)h << 16) | at;
+      } else {
+        wl[idx] = kEmSkip;
+        emit_direct(cursor, start, entries, C, (int)(unsigned)w, particle(c0 + (long long)(w >> 32)));
+      }
+    }
   }
   __syncthreads();
-  // one reservation per tile of the workgroup
-  for (int k = tid; k < kHbCap; k += kHbThreads) {
-    const int key = keys[k];
-    if (key >= 0) {
-      const unsigned long long b0 = atomicAdd(&cursor[key], (unsigned long long)vals[k]);
-      base[k] = b0;
-      if (b0 + vals[k] > start[key + 1]) set_overflow(C);
-    }
-    vals[k] = 0u;
-  }
+  reserve_round<kHbCap, kHbThreads>(keys, vals, base, cursor, start, C);
   __syncthreads();
-  const int ne = s_n;
-  for (int j = tid; j < ne; j += kHbThreads) {
-    const unsigned v = list[j];
-    const int h = (int)(v >> 9);
-    const int key = keys[h];
-    const unsigned long long pos = base[h] + atomicAdd(&vals[h], 1u);
-    if (pos < start[key + 1]) entries[pos] = parts[v & 511u];
+  for (int idx = lane; idx < nw; idx += 64) {
+    const unsigned long long w = wl[idx];
+    if (w == kEmSkip) continue;
+    const int h = (int)((w >> 16) & 0xffffu);
+    const unsigned long long pos = base[h] + (w & 0xffffu);
+    if (pos < start[keys[h] + 1]) entries[pos] = particle(c0 + (long long)(w >> 32));
   }
   // per-workgroup partials (members of the base grid, largest mass)
   valid = wave_sum(valid);
@@ -206,133 +322,198 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
 }
 
 // The emission pass of one level: items [item_lo, item_hi) are the work items
-// (chunks of one tile's entry range) of the level's tiles.
-__global__ void __launch_bounds__(kEmThreads) emit_tsc(Plan P, const Item* items, long long item_lo,
-                                                       long long item_hi, unsigned long long* cursor,
+// (chunks of one tile's entry range) of the level's tiles.  A workgroup takes
+// `batch` consecutive items at a time from the level's queue counter and
+// works through those whose tile has targets (the others' member counts stay
+// 0).  Per item the target records are staged in LDS (kEmTargets at a time);
+// the item's entries are taken kEmRound per round, kEmPer per thread.  In a
+// round every wave tests its entries against the targets and lists its
+// member emissions (destination tile key, entry) in its own LDS list, then
+// counts the list 64 emissions at a time (a position in the key's range of
+// the round for each, written back over the list word); one global add per
+// key reserves the round's ranges and every wave writes its listed entries,
+// reading each again from its source tile.
+__global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* items, long long item_lo,
+                                                       long long item_hi, int level, int batch,
+                                                       unsigned long long* cursor,
                                                        const unsigned long long* start, int4* entries,
                                                        unsigned* item_pairs, Counters* C) {
   __shared__ int keys[kEmCap];
   __shared__ unsigned vals[kEmCap];
   __shared__ unsigned long long base[kEmCap];
-  __shared__ unsigned list[kEmList];
-  __shared__ int4 stage[kEmRound];
+  __shared__ unsigned long long wlist_all[(kEmThreads / 64) * kEmWList];
+  __shared__ unsigned mlist_all[(kEmThreads / 64) * 128];
   __shared__ BoxRec recs[kEmTargets];
-  __shared__ int s_n;
+  __shared__ unsigned smask[64];   // per sub-block of the tile: the targets that can hold its members
+  __shared__ long long s_i0;
   __shared__ unsigned s_red[kEmThreads / 64];
-  const int tid = threadIdx.x;
-  for (long long ii = item_lo + blockIdx.x; ii < item_hi; ii += gridDim.x) {
-    const Item it = items[ii];
-    const int t = it.tile;
-    const unsigned q0 = P.tgt_off[t], q1 = P.tgt_off[t + 1];
-    if (q0 == q1) {
-      if (tid == 0) item_pairs[ii] = 0u;
-      continue;
-    }
-    const int g = P.tile_grid[t];
-    const GridDesc& G = P.grids[g];
-    const int gn[3] = {G.n[0], G.n[1], G.n[2]};
-    const float gorg[3] = {G.org[0], G.org[1], G.org[2]};
-    const float gd[3] = {G.d[0], G.d[1], G.d[2]};
-    const float gid[3] = {G.inv_d[0], G.inv_d[1], G.inv_d[2]};
-    const int local = t - G.tile0;
-    const int tc[3] = {local % G.nt[0], (local / G.nt[0]) % G.nt[1], local / (G.nt[0] * G.nt[1])};
-    const long long e0 = it.begin;
-    const long long e1 = min(it.end, (long long)cursor[2 * t]);
-    for (int k = tid; k < kEmCap; k += kEmThreads) { keys[k] = -1; vals[k] = 0u; }
-    unsigned pairs = 0;
-    for (long long r0 = e0; r0 < e1; r0 += kEmRound) {
-      const int nr = (int)min((long long)kEmRound, e1 - r0);
-      if (tid == 0) s_n = 0;
-      for (int k = tid; k < nr; k += kEmThreads) stage[k] = entries[r0 + k];
-      __syncthreads();
+  const int tid = threadIdx.x, lane = (int)__lane_id();
+  unsigned long long* wl = wlist_all + (tid >> 6) * kEmWList;
+  unsigned* ml = mlist_all + (tid >> 6) * 128;
+  unsigned* queue = &C->equeue[level * kLine];
+  for (;;) {
+    if (tid == 0) s_i0 = item_lo + (long long)atomicAdd(queue, (unsigned)batch);
+    __syncthreads();
+    const long long i0 = s_i0;
+    __syncthreads();   // every wave has read s_i0 before thread 0 takes the next batch
+    if (i0 >= item_hi) break;
+    const int nb = (int)min((long long)batch, item_hi - i0);
+    for (int bi = 0; bi < nb; ++bi) {
+      const long long ii = i0 + bi;
+      const Item it = items[ii];
+      const int t = it.tile;
+      const unsigned q0 = P.tgt_off[t], q1 = P.tgt_off[t + 1];
+      if (q0 == q1) continue;
+      const int g = P.tile_grid[t];
+      const GridDesc& G = P.grids[g];
+      const int gn[3] = {G.n[0], G.n[1], G.n[2]};
+      const float gorg[3] = {G.org[0], G.org[1], G.org[2]};
+      const float gid[3] = {G.inv_d[0], G.inv_d[1], G.inv_d[2]};
+      const int local = t - G.tile0;
+      const int tc[3] = {local % G.nt[0], (local / G.nt[0]) % G.nt[1], local / (G.nt[0] * G.nt[1])};
+      const long long e0 = it.begin;
+      const long long e1 = min(it.end, (long long)cursor[2 * t]);
+      for (int k = tid; k < kEmCap; k += kEmThreads) { keys[k] = -1; vals[k] = 0u; }
+      unsigned pairs = 0;
       for (unsigned q = q0; q < q1; q += kEmTargets) {
         const int nq = (int)min((unsigned)kEmTargets, q1 - q);
         for (int k = tid; k < nq * 8; k += kEmThreads)
           reinterpret_cast<int4*>(recs)[k] = reinterpret_cast<const int4*>(P.grec + P.tgt[q + (k >> 3)])[k & 7];
+        if (tid < 64) smask[tid] = 0u;
         __syncthreads();
-        for (int k = tid; k < nr; k += kEmThreads) {
-          const int4 e = stage[k];
-          const float p[3] = {__int_as_float(e.x), __int_as_float(e.y), __int_as_float(e.z)};
-          // home: the clamped cell of this grid lies in this tile
-          bool home = true;
+        // sub-block (sx, sy, sz) of the tile, 4 per axis, holds the home
+        // cells a..b per axis; a member homed there lies within two cells of
+        // them, so every target whose box meets that widened range is listed
+        for (int w = tid; w < 64 * kEmTargets; w += kEmThreads) {
+          const int sb = w / kEmTargets, j = w % kEmTargets;
+          if (j >= nq) continue;
+          const BoxRec& R = recs[j];
+          bool meet = true;
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            int ci;
-            if (g == 0) ci = tsc_base_index(p[a], P.xl, gd[a], gid[a], gn[a]);
-            else ci = clampi((int)index_quotient(p[a] - gorg[a], gd[a], gid[a]), 1, gn[a]);
-            home = home && ((ci - 1) >> P.tshift[a]) == tc[a];
+            const int sh = P.tshift[a] - 2;
+            const int ca = (tc[a] << P.tshift[a]) + (((sb >> (2 * a)) & 3) << sh) + 1;
+            const int cb = ca + (1 << sh) - 1;
+            const float o = g == 0 ? P.xl : gorg[a];
+            meet = meet && R.lo[a] < o + (float)(cb + 2) * G.d[a] && R.hi[a] > o + (float)(ca - 3) * G.d[a];
           }
-          if (!home) continue;
-          for (int j = 0; j < nq; ++j) {
-            const BoxRec& R = recs[j];
-            if (!in_box(R, p[0], p[1], p[2])) continue;
-            ++pairs;
-            int lo[3], hi[3];
-            bool empty = false;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const int ci = (int)index_quotient(p[a] - R.org[a], R.d[a], R.inv_d[a]);
-              const int c0 = max(ci - 1, 1), c1 = min(ci + 1, R.n[a]);
-              // a member whose quotient rounds onto the far face of the box
-              // (ci = N + 2) has no stencil cell in the patch
-              empty = empty || c1 < c0;
-              lo[a] = (c0 - 1) >> P.tshift[a];
-              hi[a] = (c1 - 1) >> P.tshift[a];
-            }
-            if (empty) continue;
-            for (int cz = lo[2]; cz <= hi[2]; ++cz)
-              for (int cy = lo[1]; cy <= hi[1]; ++cy)
-                for (int cx = lo[0]; cx <= hi[0]; ++cx) {
-                  const int key = 2 * (R.tile0 + (cz * R.nt1 + cy) * R.nt0 + cx);
-                  const int h = slot_insert<kEmCap>(keys, key);
-                  int pos = -1;
-                  if (h >= 0) {
-                    pos = atomicAdd(&s_n, 1);
-                    if (pos < kEmList) {
-                      atomicAdd(&vals[h], 1u);
-                      list[pos] = ((unsigned)h << 10) | (unsigned)k;   // h < 1024, k < 1024
-                    }
-                  }
-                  if (h < 0 || pos >= kEmList) emit_direct(cursor, start, entries, C, key, e);
-                }
-          }
+          if (meet) atomicOr(&smask[sb], 1u << j);
         }
         __syncthreads();
-      }
-      // one reservation per tile key of the round
-      for (int k = tid; k < kEmCap; k += kEmThreads) {
-        const int key = keys[k];
-        const unsigned v = vals[k];
-        if (key >= 0 && v) {
-          const unsigned long long b0 = atomicAdd(&cursor[key], (unsigned long long)v);
-          base[k] = b0;
-          if (b0 + v > start[key + 1]) set_overflow(C);
+        for (long long r0 = e0; r0 < e1; r0 += kEmRound) {
+          // ---- list the wave's member emissions
+          int nw = 0;   // the wave's list length (uniform)
+#pragma unroll 1
+          for (int i = 0; i < kEmPer; ++i) {
+            const int k = i * kEmThreads + tid;
+            const bool ok = r0 + k < e1;
+            const int4 v = ok ? entries[r0 + k] : make_int4(0, 0, 0, 0);
+            const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
+            // home: the clamped cell of this grid lies in this tile
+            bool home = ok;
+            int sb = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const int hc = home_cell(p[a], g == 0 ? P.xl : gorg[a], gid[a], g == 0, gn[a]) - 1;
+              home = home && (hc >> P.tshift[a]) == tc[a];
+              sb |= ((hc >> (P.tshift[a] - 2)) & 3) << (2 * a);
+            }
+            // the targets of the entry's sub-block, one per pass: the box
+            // tests, with the members listed (lane, target) in the wave's
+            // member list; every 64 listed members, and at the end, the wave
+            // works through them one per lane
+            int nm = 0;   // the member list's length (uniform)
+            auto members = [&](int n) {
+              const int src = (int)(ml[lane < n ? lane : 0] & 63u);
+              const int j = (int)(ml[lane < n ? lane : 0] >> 8);
+              const int4 u = make_int4(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src), __shfl(v.w, src));
+              const float q[3] = {__int_as_float(u.x), __int_as_float(u.y), __int_as_float(u.z)};
+              const BoxRec& R = recs[j];
+              bool mem = lane < n;
+              int lo[3] = {0, 0, 0}, ext[3] = {1, 1, 1};
+              if (mem) {
+                ++pairs;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                  const int ci = (int)index_quotient(q[a] - R.org[a], R.d[a], R.inv_d[a]);
+                  const int c0 = max(ci - 1, 1), c1 = min(ci + 1, R.n[a]);
+                  // a member whose quotient rounds onto the far face of the
+                  // box (ci = N + 2) has no stencil cell in the patch
+                  mem = mem && c0 <= c1;
+                  lo[a] = (c0 - 1) >> P.tshift[a];
+                  ext[a] = ((c1 - 1) >> P.tshift[a]) - lo[a] + 1;
+                }
+              }
+              const int nt = mem ? ext[0] * ext[1] * ext[2] : 0;
+              const int ks = i * kEmThreads + (tid & ~63) + src;
+              int tot;
+              const int off = nw + wave_prefix4(nt, tot);
+              for (int c = 0; c < nt; ++c) {
+                const int r = div_small(c, ext[0]), rz = div_small(r, ext[1]);
+                const int cx = lo[0] + c - r * ext[0], cy = lo[1] + r - rz * ext[1], cz = lo[2] + rz;
+                const int key = 2 * (R.tile0 + (cz * R.nt1 + cy) * R.nt0 + cx);
+                if (off + c < kEmWList) wl[off + c] = (unsigned long long)(unsigned)key | ((unsigned long long)ks << 32);
+                else emit_direct(cursor, start, entries, C, key, u);
+              }
+              nw = min(nw + tot, kEmWList);
+            };
+            for (unsigned tm = home ? smask[sb] : 0u; __ballot(tm != 0u);) {
+              const int j = tm ? __ffs(tm) - 1 : 0;
+              const bool has = tm != 0u;
+              tm &= tm - 1u;
+              const bool mem = has && in_box(recs[j], p[0], p[1], p[2]);
+              const unsigned long long mb = __ballot(mem);
+              if (mem) ml[nm + __popcll(mb & ((1ull << lane) - 1ull))] = (unsigned)lane | ((unsigned)j << 8);
+              nm += __popcll(mb);
+              if (nm >= 64) {
+                members(64);
+                nm -= 64;
+                if (lane < nm) ml[lane] = ml[64 + lane];
+              }
+            }
+            if (nm > 0) members(nm);
+          }
+          // ---- count the list: a position in the key's range of the round
+          for (int b0 = 0; b0 < nw; b0 += 64) {
+            const int idx = b0 + lane;
+            const bool want = idx < nw;
+            const unsigned long long w = want ? wl[idx] : 0ull;
+            int h;
+            unsigned at;
+            count_round<kEmCap>(keys, vals, want, (int)(unsigned)w, h, at);
+            if (want) {
+              if (h >= 0) {
+                wl[idx] = (w & 0xffffffff00000000ull) | ((unsigned long long)h << 16) | at;
+              } else {
+                wl[idx] = kEmSkip;
+                emit_direct(cursor, start, entries, C, (int)(unsigned)w, entries[r0 + (long long)(w >> 32)]);
+              }
+            }
+          }
+          __syncthreads();
+          reserve_round<kEmCap, kEmThreads>(keys, vals, base, cursor, start, C);
+          __syncthreads();
+          // ---- write the listed entries
+          for (int idx = lane; idx < nw; idx += 64) {
+            const unsigned long long w = wl[idx];
+            if (w == kEmSkip) continue;
+            const int h = (int)((w >> 16) & 0xffffu);
+            const unsigned long long pos = base[h] + (w & 0xffffu);
+            if (pos < start[keys[h] + 1]) entries[pos] = entries[r0 + (long long)(w >> 32)];
+          }
         }
-        vals[k] = 0u;
+      }
+      // members found by this item
+      const unsigned sw = wave_sum(pairs);
+      if ((tid & 63) == 0) s_red[tid >> 6] = sw;
+      __syncthreads();
+      if (tid == 0) {
+        unsigned tot = 0;
+        for (int w = 0; w < kEmThreads / 64; ++w) tot += s_red[w];
+        item_pairs[ii] = tot;
       }
       __syncthreads();
-      const int ne = min(s_n, kEmList);
-      for (int j = tid; j < ne; j += kEmThreads) {
-        const unsigned v = list[j];
-        const int h = (int)(v >> 10);
-        const int key = keys[h];
-        const unsigned long long pos = base[h] + atomicAdd(&vals[h], 1u);
-        if (pos < start[key + 1]) entries[pos] = stage[v & 1023u];
-      }
-      __syncthreads();
-      for (int k = tid; k < kEmCap; k += kEmThreads) vals[k] = 0u;
-      __syncthreads();
     }
-    // members found by this item
-    unsigned s = wave_sum(pairs);
-    if ((tid & 63) == 0) s_red[tid >> 6] = s;
-    __syncthreads();
-    if (tid == 0) {
-      unsigned tot = 0;
-      for (int w = 0; w < kEmThreads / 64; ++w) tot += s_red[w];
-      item_pairs[ii] = tot;
-    }
-    __syncthreads();
   }
 }
 
@@ -383,12 +564,24 @@ hipError_t launch_bin_base_tsc(const Plan& P, const Parts& A, unsigned long long
   return hipGetLastError();
 }
 
-hipError_t launch_emit_tsc(const Plan& P, const Item* items, long long item_lo, long long item_hi,
+hipError_t launch_emit_tsc(const Plan& P, const Item* items, long long item_lo, long long item_hi, int level,
                            unsigned long long* cursor, const unsigned long long* start, int4* entries,
                            unsigned* item_pairs, Counters* C, int cus, hipStream_t s) {
   if (item_hi <= item_lo) return hipSuccess;
-  const int grid = (int)std::min<long long>(item_hi - item_lo, (long long)cus * 8);
-  emit_tsc<<<grid, kEmThreads, 0, s>>>(P, items, item_lo, item_hi, cursor, start, entries, item_pairs, C);
+  if (level < 0 || level >= kNLevels) return hipErrorInvalidValue;
+  // the workgroups that fit the chip at once; the queue balances the items,
+  // taken up to 8 at a time while there are many per workgroup
+  static int per_cu = 0;
+  if (per_cu <= 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, emit_tsc, kEmThreads, 0) != hipSuccess || nb <= 0) nb = 4;
+    per_cu = nb;
+  }
+  const long long n = item_hi - item_lo;
+  const int grid = (int)std::min<long long>(n, (long long)cus * per_cu);
+  const int batch = (int)std::max<long long>(1, std::min<long long>(8, n / ((long long)grid * 16)));
+  emit_tsc<<<grid, kEmThreads, 0, s>>>(P, items, item_lo, item_hi, level, batch, cursor, start, entries, item_pairs,
+                                       C);
   return hipGetLastError();
 }
 

@@ -138,6 +138,7 @@ struct Counters {
   unsigned long long pool_fill[kPoolSubs * kLine / 2];   // sub-pool k at [k * kLine / 2]
   unsigned long long epool_fill[kPoolSubs * kLine / 2];  // emission lists, same layout
   unsigned int queue[kQueues * kLine];                   // queue q at [q * kLine]
+  unsigned int equeue[kNLevels * kLine];                 // emission pass of level l at [l * kLine]
 };
 
 static_assert(sizeof(unsigned long long) * 4 + sizeof(unsigned) * (kLine - 8) == kLine * 4,
