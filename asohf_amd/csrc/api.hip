@@ -793,7 +793,8 @@ static asohf_status prepare_args(asohf_ctx* ctx, const asohf_opts& o, const Part
   a.nkeys = 2ll * a.ntiles;
   a.red = o.reduction;
   a.elem = a.red == 0 ? 8 : 4;
-  a.nblk = bin_blocks(A.n);
+  // the per-block tables serve both binning forms: sized for the larger
+  a.nblk = std::max(bin_blocks(A.n), hier_bin_blocks(A.n));
   const long long nkeys = a.nkeys, nblk = a.nblk;
   const int ntiles = a.ntiles;
   HIPCHK(ctx->cnt.ensure(sizeof(unsigned) * std::max<long long>(nkeys, 1)), "allocating counts");
@@ -889,7 +890,7 @@ static asohf_status enqueue_count(asohf_ctx* ctx, const PipeArgs& a, hipStream_t
   HIPCHK(launch_fill_u32((unsigned*)a.C, 0u, sizeof(Counters) / 4, s), "clearing counters");
   HIPCHK(launch_fill_u32(a.cnt, 0u, a.nkeys, s), "clearing counts");
   HIPCHK(launch_count(a.P, a.A, a.cnt, a.C, a.BT, a.cus, s), "count kernel");
-  if (a.nblk > 0) HIPCHK(launch_reduce_blocks(a.BT, a.nblk, a.C, s), "block sums");
+  if (a.A.n > 0) HIPCHK(launch_reduce_blocks(a.BT, bin_blocks(a.A.n), a.C, s), "block sums");
   HIPCHK(launch_scan(a.cnt, a.nkeys, a.bsum, a.start, a.cursor, s), "scan");
   HIPCHK(launch_plan_count(a.start, a.ntiles, a.chunk, a.chunk_wide, a.nitems, a.nsplit, a.is_split, s),
          "planning items");
@@ -1069,7 +1070,7 @@ static asohf_status enqueue_hier(asohf_ctx* ctx, const PipeArgs& a, hipStream_t 
   HIPCHK(launch_cursor_reset(a.start, a.cursor, a.nkeys, a.cus, s), "cursors");
   HIPCHK(launch_bin_base_tsc(a.P, a.A, a.cursor, a.start, const_cast<int4*>(a.D.entries), a.C, a.BT, s),
          "base-grid binning");
-  if (a.nblk > 0) HIPCHK(launch_reduce_blocks(a.BT, a.nblk, a.C, s), "block sums");
+  if (a.A.n > 0) HIPCHK(launch_reduce_blocks(a.BT, hier_bin_blocks(a.A.n), a.C, s), "block sums");
   if (events) {
     HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
     HIPCHK(hipEventRecord(ctx->ev[2], s), "event");

@@ -11,6 +11,8 @@ namespace asohf {
 
 hipError_t launch_fill_u32(unsigned* p, unsigned v, long long n, hipStream_t s);
 long long bin_blocks(long long n);
+// workgroups of the hierarchy path's base-grid binning (its per-block tables)
+long long hier_bin_blocks(long long n);
 hipError_t launch_count(const Plan& P, const Parts& A, unsigned* cnt, Counters* C, const BlockTables& BT,
                         int cus, hipStream_t s);
 hipError_t launch_scatter(const Plan& P, const Parts& A, unsigned long long* cursor, int4* entries,
