@@ -1003,10 +1003,13 @@ static asohf_status hier_plan(asohf_ctx* ctx, PipeArgs& a, hipStream_t s) {
   HostPlan& H = ctx->plan;
   HIPCHK(launch_fill_u32((unsigned*)a.C, 0u, sizeof(Counters) / 4, s), "clearing counters");
   HIPCHK(launch_fill_u32(a.cnt, 0u, a.nkeys, s), "clearing counts");
-  HIPCHK(launch_count(a.P, a.A, a.cnt, a.C, a.BT, a.cus, s), "count kernel");
+  // a member's home-tile entry counts under key 2t, its others under 2t + 1
+  Plan pc = a.P;
+  pc.home_split = 1;
+  HIPCHK(launch_count(pc, a.A, a.cnt, a.C, a.BT, a.cus, s), "count kernel");
   HIPCHK(launch_count_to_capacity(a.cnt, a.nkeys, a.cus, s), "capacities");
   HIPCHK(launch_scan(a.cnt, a.nkeys, a.bsum, a.start, a.cursor, s), "scan");
-  HIPCHK(launch_plan_count(a.start, a.ntiles, a.chunk, a.chunk_wide, a.nitems, a.nsplit, a.is_split, s),
+  HIPCHK(launch_plan_count(a.start, a.ntiles, a.chunk, 0, a.nitems, a.nsplit, a.is_split, s),
          "planning items");
   HIPCHK(launch_scan(a.nitems, a.ntiles, a.bsum, a.item_off, nullptr, s), "item offsets");
   HIPCHK(launch_scan(a.nsplit, a.ntiles, a.bsum, a.split_off, nullptr, s), "split offsets");
@@ -1037,7 +1040,7 @@ static asohf_status hier_plan(asohf_ctx* ctx, PipeArgs& a, hipStream_t s) {
   a.D.entries = ctx->entries.as<int4>();
   a.D.items = ctx->items.as<Item>();
   a.D.scratch = ctx->scratch.p;
-  HIPCHK(launch_plan_write(a.start, a.ntiles, a.chunk, a.chunk_wide, a.item_off, a.split_off, a.split_rank,
+  HIPCHK(launch_plan_write(a.start, a.ntiles, a.chunk, 0, a.item_off, a.split_off, a.split_rank,
                            const_cast<Item*>(a.D.items), a.split, a.split_tiles, a.D.tile_mass_i, a.C, s),
          "writing items");
   ctx->hier_nitems = n_items;
@@ -1051,6 +1054,7 @@ static void hier_bind(asohf_ctx* ctx, PipeArgs& a) {
   a.D.items = ctx->items.as<Item>();
   a.D.scratch = ctx->scratch.p;
   a.D.cursor = a.cursor;
+  a.D.start = a.start;
   a.item_pairs = ctx->ipairs.as<unsigned>();
   a.n_items = ctx->hier_nitems;
   for (int l = 0; l <= kNLevels + 1; ++l) a.lvl_items[l] = ctx->hier_items[std::min(l, a.P.nl + 1)];

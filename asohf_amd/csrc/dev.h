@@ -119,6 +119,18 @@ __device__ __forceinline__ float index_quotient(float a, float d, float inv_d) {
 // MINIRADX(II) = PATCHRX + (FLOAT(II)-1.5)*DXPA  (grids.f:1304-1312)
 __device__ __forceinline__ float minirad(float pr, int ii, float d) { return pr + ((float)ii - 1.5f) * d; }
 
+// The cell that decides a member's home tile: INT((x - origin) / d) + 1
+// clamped (the base grid) or INT((x - origin) / d) clamped (a patch) with
+// the quotient taken as a product with the reciprocal.  It is within one cell
+// of the reference's index, so it lies in the member's stencil and its tile
+// holds one of the member's entries; every entry of the member computes the
+// same value, so exactly one of them is home.
+__device__ __forceinline__ int home_cell(float x, float org, float inv_d, bool base, int n) {
+  float t = (x - org) * inv_d;
+  t = fminf(fmaxf(t, -1.0e9f), 1.0e9f);
+  return clampi((int)t + (base ? 1 : 0), 1, n);
+}
+
 // INT((XP-XL)/DX)+1 clamped, on the base grid; t is guarded so the
 // conversion is always defined.
 __device__ __forceinline__ int tsc_base_index(float xp, float xl, float d, float inv_d, int n) {
@@ -316,10 +328,20 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
     int4 e;
     if (MODE == 2) e = make_int4(__float_as_int(x), __float_as_int(y), __float_as_int(z), __float_as_int(m));
     else e = pack_table(ix, jy, kz, 0, m);
+    // the hierarchy binning's split: the home tile's entry under 2t, the
+    // others under 2t + 1
+    int hb[3] = {-1, -1, -1};
+    if (MODE == 2 && P.home_split) {
+      hb[0] = (home_cell(x, P.xl, bid[0], true, bn[0]) - 1) >> P.tshift[0];
+      hb[1] = (home_cell(y, P.xl, bid[1], true, bn[1]) - 1) >> P.tshift[1];
+      hb[2] = (home_cell(z, P.xl, bid[2], true, bn[2]) - 1) >> P.tshift[2];
+    }
     for (int c = 0; c < nz_; ++c)
       for (int b = 0; b < ny_; ++b)
-        for (int a = 0; a < nx_; ++a)
-          emit(0, btile0 + (tz[c] * bnt1 + ty[b]) * bnt0 + tx[a], 0, e);
+        for (int a = 0; a < nx_; ++a) {
+          const int away = (MODE == 2 && P.home_split && !(tx[a] == hb[0] && ty[b] == hb[1] && tz[c] == hb[2]));
+          emit(0, btile0 + (tz[c] * bnt1 + ty[b]) * bnt0 + tx[a], away, e);
+        }
   }
   // ---------------- patches, level by level ----------------
   // When every patch's membership box lies inside a box of the level above
@@ -367,10 +389,18 @@ __device__ __forceinline__ void enumerate_particle(const Plan& P, long long i, f
       const int t0x = (lo[0] - 1) >> P.tshift[0], t1x = (hi[0] - 1) >> P.tshift[0];
       const int t0y = (lo[1] - 1) >> P.tshift[1], t1y = (hi[1] - 1) >> P.tshift[1];
       const int t0z = (lo[2] - 1) >> P.tshift[2], t1z = (hi[2] - 1) >> P.tshift[2];
+      int hp[3] = {-1, -1, -1};
+      if (MODE == 2 && P.home_split) {
+        hp[0] = (home_cell(x, gorg[0], gid[0], false, gn[0]) - 1) >> P.tshift[0];
+        hp[1] = (home_cell(y, gorg[1], gid[1], false, gn[1]) - 1) >> P.tshift[1];
+        hp[2] = (home_cell(z, gorg[2], gid[2], false, gn[2]) - 1) >> P.tshift[2];
+      }
       for (int c = t0z; c <= t1z; ++c)
         for (int b = t0y; b <= t1y; ++b)
-          for (int a = t0x; a <= t1x; ++a)
-            emit(g, gtile0 + (c * gnt1 + b) * gnt0 + a, wide, e);
+          for (int a = t0x; a <= t1x; ++a) {
+            const int away = (MODE == 2 && P.home_split && !(a == hp[0] && b == hp[1] && c == hp[2]));
+            emit(g, gtile0 + (c * gnt1 + b) * gnt0 + a, wide | away, e);
+          }
     }
     if (!any && P.nested) break;
   }

@@ -82,18 +82,6 @@ __device__ __forceinline__ int div_small(int c, int e) {
   return (c * (e == 1 ? 256 : (e == 2 ? 128 : 86))) >> 8;
 }
 
-// The cell that decides a member's home tile: INT((x - origin) / d) + 1
-// clamped (the base grid) or INT((x - origin) / d) clamped (a patch) with
-// the quotient taken as a product with the reciprocal.  It is within one cell
-// of the reference's index, so it lies in the member's stencil and its tile
-// holds one of the member's entries; every entry of the member computes the
-// same value, so exactly one of them is home.
-__device__ __forceinline__ int home_cell(float x, float org, float inv_d, bool base, int n) {
-  float t = (x - org) * inv_d;
-  t = fminf(fmaxf(t, -1.0e9f), 1.0e9f);
-  return clampi((int)t + (base ? 1 : 0), 1, n);
-}
-
 __device__ __forceinline__ void set_overflow(Counters* C) {
   C->overflow = 1u;
   C->n_items_eff = 0u;   // the deposit of this call then does nothing
@@ -253,6 +241,7 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
     }
     int tx[3] = {0, 0, 0}, ty[3] = {0, 0, 0}, tz[3] = {0, 0, 0};
     int nt = 0, nx_ = 1, ny_ = 1;
+    int hb[3] = {0, 0, 0};   // the home tile
     if (ok) {
       ++valid;
       mmax = fmaxf(mmax, m);
@@ -260,6 +249,9 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
       ny_ = base_axis_tiles(tsc_base_index(y, P.xl, bd[1], bid[1], bn[1]), bn[1], P.tshift[1], ty);
       const int nz_ = base_axis_tiles(tsc_base_index(z, P.xl, bd[2], bid[2], bn[2]), bn[2], P.tshift[2], tz);
       nt = nx_ * ny_ * nz_;
+      hb[0] = (home_cell(x, P.xl, bid[0], true, bn[0]) - 1) >> P.tshift[0];
+      hb[1] = (home_cell(y, P.xl, bid[1], true, bn[1]) - 1) >> P.tshift[1];
+      hb[2] = (home_cell(z, P.xl, bid[2], true, bn[2]) - 1) >> P.tshift[2];
     }
     int tot;
     const int off = nw + wave_prefix5(nt, tot);
@@ -269,7 +261,7 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
       const int ta = a == 0 ? tx[0] : (a == 1 ? tx[1] : tx[2]);
       const int tb = b == 0 ? ty[0] : (b == 1 ? ty[1] : ty[2]);
       const int tcz = qz == 0 ? tz[0] : (qz == 1 ? tz[1] : tz[2]);
-      const int key = 2 * ((tcz * bnt1 + tb) * bnt0 + ta);
+      const int key = 2 * ((tcz * bnt1 + tb) * bnt0 + ta) + ((ta == hb[0] && tb == hb[1] && tcz == hb[2]) ? 0 : 1);
       if (off + c < kHbWList) wl[off + c] = (unsigned long long)(unsigned)key | ((unsigned long long)k << 32);
       else emit_direct(cursor, start, entries, C, key, particle(c0 + k));
     }
@@ -371,8 +363,11 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
       const float gid[3] = {G.inv_d[0], G.inv_d[1], G.inv_d[2]};
       const int local = t - G.tile0;
       const int tc[3] = {local % G.nt[0], (local / G.nt[0]) % G.nt[1], local / (G.nt[0] * G.nt[1])};
+      // the item's part of the tile's home entries: every member of this
+      // grid has exactly one, so each member is visited once
       const long long e0 = it.begin;
-      const long long e1 = min(it.end, (long long)cursor[2 * t]);
+      const long long e1 = min(min(it.end, (long long)start[2 * t + 1]), (long long)cursor[2 * t]);
+      if (e1 <= e0) continue;
       for (int k = tid; k < kEmCap; k += kEmThreads) { keys[k] = -1; vals[k] = 0u; }
       unsigned pairs = 0;
       for (unsigned q = q0; q < q1; q += kEmTargets) {
@@ -409,13 +404,12 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
             const bool ok = r0 + k < e1;
             const int4 v = ok ? entries[r0 + k] : make_int4(0, 0, 0, 0);
             const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
-            // home: the clamped cell of this grid lies in this tile
-            bool home = ok;
+            // the entry's sub-block: where its home cell lies in the tile
+            const bool home = ok;
             int sb = 0;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               const int hc = home_cell(p[a], g == 0 ? P.xl : gorg[a], gid[a], g == 0, gn[a]) - 1;
-              home = home && (hc >> P.tshift[a]) == tc[a];
               sb |= ((hc >> (P.tshift[a] - 2)) & 3) << (2 * a);
             }
             // the targets of the entry's sub-block, one per pass: the box
@@ -445,13 +439,17 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
                 }
               }
               const int nt = mem ? ext[0] * ext[1] * ext[2] : 0;
+              int hp[3] = {0, 0, 0};   // the member's home tile in the patch
+#pragma unroll
+              for (int a = 0; a < 3; ++a) hp[a] = (home_cell(q[a], R.org[a], R.inv_d[a], false, R.n[a]) - 1) >> P.tshift[a];
               const int ks = i * kEmThreads + (tid & ~63) + src;
               int tot;
               const int off = nw + wave_prefix4(nt, tot);
               for (int c = 0; c < nt; ++c) {
                 const int r = div_small(c, ext[0]), rz = div_small(r, ext[1]);
                 const int cx = lo[0] + c - r * ext[0], cy = lo[1] + r - rz * ext[1], cz = lo[2] + rz;
-                const int key = 2 * (R.tile0 + (cz * R.nt1 + cy) * R.nt0 + cx);
+                const int key = 2 * (R.tile0 + (cz * R.nt1 + cy) * R.nt0 + cx) +
+                                ((cx == hp[0] && cy == hp[1] && cz == hp[2]) ? 0 : 1);
                 if (off + c < kEmWList) wl[off + c] = (unsigned long long)(unsigned)key | ((unsigned long long)ks << 32);
                 else emit_direct(cursor, start, entries, C, key, u);
               }

@@ -98,6 +98,8 @@ struct Plan {
   int rad_malla;             // 1: RAD tables are exactly MALLA's recurrence (rad1 + (i-1)*radc)
   int merge_narrow;          // 1: narrow deposits may merge lanes (every non-centre weight product
                              // <= 0.45, so four terms of a mass below 2^31 sum below 2^32)
+  int home_split;            // TSC count pass of the hierarchy binning: a member's entry in its
+                             // home tile t counts under key 2t, its other entries under 2t + 1
   float rad1[3];
   float radc[3];
   // TSC binning along the hierarchy: the patches whose members are found
@@ -188,9 +190,12 @@ struct DepArgs {
   long long* tile_mass_i;    // deterministic: exact fixed-point mass per tile
   double* tile_mass_d;       // atomic: FP64 mass per tile
   int n_queues;              // queues in use (<= kQueues, <= grid)
-  // TSC binning along the hierarchy: items cover capacity ranges and
-  // cursor[2 t] is the end of tile t's filled entries (NULL otherwise)
+  // TSC binning along the hierarchy: tile t's range [start[2t], start[2t+2])
+  // holds its members' home entries from start[2t] to cursor[2t] and their
+  // other entries from start[2t+1] to cursor[2t+1]; items cover capacity
+  // ranges (both NULL otherwise: every item's range is filled)
   const unsigned long long* cursor;
+  const unsigned long long* start;
   int hist_cells;            // table mode: int64 cells of the wide classes' histogram buffer
                              // (dynamic LDS, set by the launcher; 0: every class walks directly)
   int pad2;

@@ -444,9 +444,18 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply(const unsigned* in, l
 // Work items.  plan_count: items per tile and the partial-tile slots a split
 // tile needs; after exclusive scans of both, plan_write places every tile's
 // items at its offset (no atomics: the item order is the tile order).
+// With chunk_wide = 0 the tile's whole range [start[2t], start[2t+2]) is
+// chunked as narrow entries (the TSC hierarchy binning, whose keys 2t and
+// 2t + 1 are a tile's home and other entries).
 __device__ __forceinline__ void tile_chunks(const unsigned long long* start, int t, int chunk, int chunk_wide,
                                             long long& s0, long long& s1, long long& s2, int& cn, int& cw) {
   s0 = start[2 * t]; s1 = start[2 * t + 1]; s2 = start[2 * t + 2];
+  if (chunk_wide == 0) {
+    s1 = s2;
+    cn = (int)((s2 - s0 + chunk - 1) / chunk);
+    cw = 0;
+    return;
+  }
   cn = (int)((s1 - s0 + chunk - 1) / chunk);
   cw = (int)((s2 - s1 + chunk_wide - 1) / chunk_wide);
 }
@@ -591,6 +600,13 @@ __device__ __forceinline__ void deposit27(typename AccT<RED>::T* base, float m, 
     lds_add(reinterpret_cast<long long*>(base) + (1 + HX * (1 + HY * 1)),
             (long long)__float2uint_rn(ms) - (long long)rest);
   }
+}
+
+// a wave-uniform 64-bit value, moved to scalar registers
+__device__ __forceinline__ long long uniform64(long long v) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
 // the value of lane l^1 (l^2) of the wave, through a quad permutation
@@ -886,7 +902,22 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
       // loop runs the same number of passes on every lane (ok = the lane holds
       // an entry), which the lane merging of the TSC deposit needs.  An empty
       // item reads nothing: no pass runs.
-      const long long len = (D.cursor ? min(it.end, (long long)D.cursor[2 * t]) : it.end) - it.begin;
+      // the item's filled entries: [bA, bA + la) and [bB, bB + (len - la));
+      // with the hierarchy binning the tile's home entries end at cursor[2t]
+      // and its other entries run from start[2t+1] to cursor[2t+1]
+      long long la = it.end - it.begin, gap = 0, len = la;
+      if (D.cursor) {
+        const long long s1 = (long long)D.start[2 * t + 1];
+        la = max(min(min(it.end, s1), (long long)D.cursor[2 * t]) - it.begin, 0ll);
+        const long long bB = max(it.begin, s1);
+        gap = bB - (it.begin + la);
+        len = la + max(min(it.end, (long long)D.cursor[2 * t + 1]) - bB, 0ll);
+      }
+      // wave-uniform: kept in scalar registers
+      la = uniform64(la);
+      gap = uniform64(gap);
+      len = uniform64(len);
+      const int4* const eb = D.entries + uniform64(it.begin);
       auto narrow = [&](const int4 v, const bool ok) {
         int ci[3];
         float wx[3], wy[3], wz[3], m;
@@ -929,18 +960,21 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
         else if (ok)
           deposit27<RED, HX, HY>(acc + corner, m, wx, wy, wz, scale);
       };
-      const long long last = it.begin + len - 1;
+      // entry j of the item (j clamped to the last one)
+      auto at_j = [&](long long j) {
+        j = min(j, len - 1);
+        return eb[j + (j < la ? 0ll : gap)];
+      };
       int4 va = make_int4(0, 0, 0, 0), vb = va;
-      const long long e0 = it.begin + tid;
       if (len > 0) {
-        va = D.entries[min(e0, last)];
-        vb = D.entries[min(e0 + nthr, last)];
+        va = at_j(tid);
+        vb = at_j(tid + nthr);
       }
       for (long long o = 0; o < len; o += 2 * nthr) {
         narrow(va, o + tid < len);
-        va = D.entries[min(e0 + o + 2 * nthr, last)];
+        va = at_j(o + 2 * nthr + tid);
         narrow(vb, o + nthr + tid < len);
-        vb = D.entries[min(e0 + o + 3 * nthr, last)];
+        vb = at_j(o + 3 * nthr + tid);
       }
     } else if (MODE == 1) {
       // ---- wide table-mode entries (kernel level irk >= 1, half-width

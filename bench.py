@@ -427,6 +427,22 @@ def side_routines(args, bld, spec, snap, g, xyzm, kw, n, n_cells, out):
                                             "BOR 4, BORAMR 0, BOR_OVLP 0), NAMRX..Z 32, NPALEV 2^20",
                                   "routine": "CREATE_MESH (grids.f:2-707)"}
             log(f"[bench] create_mesh: {tm:.3f} s, npatch {mg.npatch.tolist()}")
+            # the density build on the hierarchy CREATE_MESH made for these
+            # particles, both modes (deterministic)
+            for md in ("tsc", "table"):
+                bld.density(mg, x, y, z, snap.npart_esp, mode=md)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    bld.density(mg, x, y, z, snap.npart_esp, mode=md)
+                torch.cuda.synchronize()
+                tb = (time.perf_counter() - t0) / args.steps
+                st_m = bld.stats()
+                out["create_mesh"][f"build_{md}"] = {"ms_per_step": 1000 * tb, "particles_per_s": n / tb,
+                                                     "n_pairs": int(st_m["n_pairs"]),
+                                                     "n_entries": int(st_m["n_entries"])}
+                log(f"[bench] build on the CREATE_MESH hierarchy ({md}): {1000 * tb:.3f} ms/step, "
+                    f"{n / tb / 1e9:.3f} G particles/s")
         except RuntimeError as err:
             out["create_mesh"] = {"error": str(err)}
             log(f"[bench] create_mesh: {err}")

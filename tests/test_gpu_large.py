@@ -4,7 +4,8 @@
     every (particle, grid) member bit-exact against the oracle's restatement of
     the reference's expressions (grids.f:1180-1192, 1294-1323) on 2^20 sampled
     particles with the whole hierarchy; the whole field (all 16.8M particles)
-    against the FP64 oracle.
+    against the FP64 oracle.  The same particles on the hierarchy CREATE_MESH
+    builds for them: membership, the whole TSC field and a kernel-table sample.
   * 512^3 particles, 6 levels (about 20,000 patches), both reductions, and
     1024^3 particles, 7 levels (about 150,000 patches), forward and adjoint:
     size-independent properties of the whole build (base mass exact,
@@ -181,6 +182,48 @@ def test_c2_membership_bitexact_and_whole_field(bld):
     assert rel_err_above_mean(u11, d11) <= 1e-5
     assert np.allclose(gm, dgm, rtol=1e-6, atol=0)
     assert float(g.fdm) != 1.0        # RENORM_DENSITY's division is exercised
+
+
+def test_c2_on_create_mesh_hierarchy(bld):
+    """The 256^3-particle workload on the hierarchy CREATE_MESH (grids.f:2-707)
+    builds for its particles (asohf_hip_create_mesh, asohf.dat's mesh block,
+    the overdensity rule of grids.f:107-129, 357-405) instead of the synthetic
+    one: TSC membership bit-exact and the whole TSC field against the FP64
+    oracle; the kernel-table DENSITY (the live path on such hierarchies) on a
+    2^18-particle sample against the FP64 oracle."""
+    spec = synth.WORKLOADS[2]
+    snap = synth.make_snapshot(spec)
+    d = torch.device("cuda:0")
+    xyzm = tuple(torch.from_numpy(a).to(d) for a in (snap.x, snap.y, snap.z, snap.m))
+    g = bld.create_mesh(snap.geom.truncated(0), *xyzm[:3], snap.npart_esp, nl_mesh=spec.nl, npalev=1 << 20)
+    assert g.nl == spec.nl and g.npatch_total > 100 and float(g.fdm) == float(snap.geom.fdm)
+    full = synth.Snapshot(snap.x, snap.y, snap.z, snap.m, snap.npart_esp, g)
+    print(f"case c2 on CREATE_MESH: {snap.n} particles, npatch {g.npatch.tolist()}", flush=True)
+    hs, dv, _ = host_sample(xyzm, snap.npart_esp, g, 1 << 18, 7)
+    got = bld.membership(g, dv[0], dv[1], dv[2], hs.npart_esp, mode="tsc").cpu().numpy()
+    want = O.pairs(hs, "B", all_members=True)
+    key = lambda a: np.lexsort((a[:, 4], a[:, 3], a[:, 2], a[:, 1], a[:, 0]))
+    assert np.array_equal(got[key(got)], want[key(want)])
+    lev = np.concatenate([[0], g.level_of_patch()])
+    assert set(np.unique(lev[got[:, 0]])) == set(range(g.nl + 1))
+    print("case c2 on CREATE_MESH: whole TSC field vs FP64", flush=True)
+    r = bld.density(g, *xyzm, snap.npart_esp, mode="tsc")
+    u1, u11, gm = r.u1.cpu().numpy(), r.u11.cpu().numpy(), r.grid_mass.cpu().numpy()
+    d1, d11, dgm = O.density64(full, "B", nthreads=0)
+    assert rel_err_above_mean(u1, d1) <= 1e-5
+    assert rel_err_above_mean(u11, d11) <= 1e-5
+    assert np.allclose(gm, dgm, rtol=1e-6, atol=0)
+    tot = float(snap.m.astype(np.float64).sum())
+    assert abs(gm[0] - tot) <= 1e-12 * tot
+    del r, d1, d11
+    print("case c2 on CREATE_MESH: kernel-table sample vs FP64", flush=True)
+    ra = bld.density(g, *dv, hs.npart_esp, mode="table")
+    a1, a11, agm = O.density64(hs, "A", nthreads=0)
+    q1, q11 = cell_unit(g, float(hs.m.max()))
+    assert rel_err_above(ra.u1.cpu().numpy(), a1, 2.0 ** 21 * q1) <= 1e-5
+    assert rel_err_above(ra.u11.cpu().numpy(), a11, 2.0 ** 21 * q11) <= 1e-5
+    assert abs(float(ra.grid_mass[0]) - float(hs.m.astype(np.float64).sum())) <= 1e-12 * float(hs.m.sum())
+    bld.release_workspaces()
 
 
 @pytest.mark.parametrize("reduction", ["deterministic", "atomic"])
