@@ -1558,8 +1558,9 @@ asohf_status asohf_hip_autotune(asohf_ctx* ctx, const asohf_geom* geom, int64_t 
   if (!ctx) return ASOHF_ERR_INVALID;
   asohf_opts o = opts ? *opts : default_opts();
   o.path = ASOHF_PATH_BATCHED;
-  // coordinate descent: tile shape, then workgroup size, then item chunk,
-  // then persistent grid; each candidate is one warm-up call and two timed ones
+  // tile shape and workgroup size together (the deposit's LDS tile and its
+  // threads go together), then item chunk, then persistent grid; each
+  // candidate is one warm-up call and two timed ones
   static const int shapes[][3] = {{8, 8, 8}, {16, 8, 8}, {16, 16, 8}, {16, 16, 16}};
   static const int blocks[] = {256, 512, 1024};
   static const int chunks[] = {2048, 8192, 32768};
@@ -1582,16 +1583,14 @@ asohf_status asohf_hip_autotune(asohf_ctx* ctx, const asohf_geom* geom, int64_t 
     return ASOHF_OK;
   };
   asohf_status st0;
-  for (auto& sh : shapes) {
-    asohf_tune t = best;
-    t.tile[0] = sh[0]; t.tile[1] = sh[1]; t.tile[2] = sh[2];
-    if ((st0 = trial(t)) != ASOHF_OK) return st0;
-  }
-  for (int bl : blocks) {
-    asohf_tune t = best;
-    t.block = bl;
-    if ((st0 = trial(t)) != ASOHF_OK) return st0;
-  }
+  const asohf_tune start = best;
+  for (auto& sh : shapes)
+    for (int bl : blocks) {
+      asohf_tune t = start;
+      t.tile[0] = sh[0]; t.tile[1] = sh[1]; t.tile[2] = sh[2];
+      t.block = bl;
+      if ((st0 = trial(t)) != ASOHF_OK) return st0;
+    }
   for (int ch : chunks) {
     asohf_tune t = best;
     t.chunk = ch;
