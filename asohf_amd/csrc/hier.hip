@@ -336,7 +336,6 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
   __shared__ unsigned long long wlist_all[(kEmThreads / 64) * kEmWList];
   __shared__ unsigned mlist_all[(kEmThreads / 64) * 128];
   __shared__ BoxRec recs[kEmTargets];
-  __shared__ unsigned smask[64];   // per sub-block of the tile: the targets that can hold its members
   __shared__ long long s_i0;
   __shared__ unsigned s_red[kEmThreads / 64];
   const int tid = threadIdx.x, lane = (int)__lane_id();
@@ -356,13 +355,6 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
       const int t = it.tile;
       const unsigned q0 = P.tgt_off[t], q1 = P.tgt_off[t + 1];
       if (q0 == q1) continue;
-      const int g = P.tile_grid[t];
-      const GridDesc& G = P.grids[g];
-      const int gn[3] = {G.n[0], G.n[1], G.n[2]};
-      const float gorg[3] = {G.org[0], G.org[1], G.org[2]};
-      const float gid[3] = {G.inv_d[0], G.inv_d[1], G.inv_d[2]};
-      const int local = t - G.tile0;
-      const int tc[3] = {local % G.nt[0], (local / G.nt[0]) % G.nt[1], local / (G.nt[0] * G.nt[1])};
       // the item's part of the tile's home entries: every member of this
       // grid has exactly one, so each member is visited once
       const long long e0 = it.begin;
@@ -374,26 +366,6 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
         const int nq = (int)min((unsigned)kEmTargets, q1 - q);
         for (int k = tid; k < nq * 8; k += kEmThreads)
           reinterpret_cast<int4*>(recs)[k] = reinterpret_cast<const int4*>(P.grec + P.tgt[q + (k >> 3)])[k & 7];
-        if (tid < 64) smask[tid] = 0u;
-        __syncthreads();
-        // sub-block (sx, sy, sz) of the tile, 4 per axis, holds the home
-        // cells a..b per axis; a member homed there lies within two cells of
-        // them, so every target whose box meets that widened range is listed
-        for (int w = tid; w < 64 * kEmTargets; w += kEmThreads) {
-          const int sb = w / kEmTargets, j = w % kEmTargets;
-          if (j >= nq) continue;
-          const BoxRec& R = recs[j];
-          bool meet = true;
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const int sh = P.tshift[a] - 2;
-            const int ca = (tc[a] << P.tshift[a]) + (((sb >> (2 * a)) & 3) << sh) + 1;
-            const int cb = ca + (1 << sh) - 1;
-            const float o = g == 0 ? P.xl : gorg[a];
-            meet = meet && R.lo[a] < o + (float)(cb + 2) * G.d[a] && R.hi[a] > o + (float)(ca - 3) * G.d[a];
-          }
-          if (meet) atomicOr(&smask[sb], 1u << j);
-        }
         __syncthreads();
         for (long long r0 = e0; r0 < e1; r0 += kEmRound) {
           // ---- list the wave's member emissions
@@ -404,18 +376,11 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
             const bool ok = r0 + k < e1;
             const int4 v = ok ? entries[r0 + k] : make_int4(0, 0, 0, 0);
             const float p[3] = {__int_as_float(v.x), __int_as_float(v.y), __int_as_float(v.z)};
-            // the entry's sub-block: where its home cell lies in the tile
-            const bool home = ok;
-            int sb = 0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const int hc = home_cell(p[a], g == 0 ? P.xl : gorg[a], gid[a], g == 0, gn[a]) - 1;
-              sb |= ((hc >> (P.tshift[a] - 2)) & 3) << (2 * a);
-            }
-            // the targets of the entry's sub-block, one per pass: the box
-            // tests, with the members listed (lane, target) in the wave's
-            // member list; every 64 listed members, and at the end, the wave
-            // works through them one per lane
+            const bool home = ok;   // the range holds the tile's home entries only
+            // the item's targets, one per pass: the box tests, with the
+            // members listed (lane, target) in the wave's member list; every
+            // 64 listed members, and at the end, the wave works through them
+            // one per lane
             int nm = 0;   // the member list's length (uniform)
             auto members = [&](int n) {
               const int src = (int)(ml[lane < n ? lane : 0] & 63u);
@@ -455,11 +420,8 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
               }
               nw = min(nw + tot, kEmWList);
             };
-            for (unsigned tm = home ? smask[sb] : 0u; __ballot(tm != 0u);) {
-              const int j = tm ? __ffs(tm) - 1 : 0;
-              const bool has = tm != 0u;
-              tm &= tm - 1u;
-              const bool mem = has && in_box(recs[j], p[0], p[1], p[2]);
+            for (int j = 0; j < nq; ++j) {
+              const bool mem = home && in_box(recs[j], p[0], p[1], p[2]);
               const unsigned long long mb = __ballot(mem);
               if (mem) ml[nm + __popcll(mb & ((1ull << lane) - 1ull))] = (unsigned)lane | ((unsigned)j << 8);
               nm += __popcll(mb);
