@@ -110,10 +110,14 @@ __device__ __forceinline__ void tsc_weights_dc(float xp, float c0, float c1, flo
 // |q|*6e-8, so when t is farther than |t|*2^-20 from every integer, RN(q) lies
 // strictly between the same two integers and truncating or flooring either
 // gives the same index; otherwise the IEEE division is taken.
+// The division runs only in waves where some lane needs it (a uniform
+// branch: computed for every lane and selected, it costs about a dozen
+// instructions per call).
 __device__ __forceinline__ float index_quotient(float a, float d, float inv_d) {
   const float t = a * inv_d;
-  if (fabsf(t - rintf(t)) > fabsf(t) * 9.5367431640625e-7f) return t;
-  return a / d;
+  const bool near = !(fabsf(t - rintf(t)) > fabsf(t) * 9.5367431640625e-7f);
+  if (__ballot(near)) return near ? a / d : t;
+  return t;
 }
 
 // MINIRADX(II) = PATCHRX + (FLOAT(II)-1.5)*DXPA  (grids.f:1304-1312)

@@ -97,7 +97,7 @@ struct Plan {
   int nested;                // 1: every level-l box lies inside a level-(l-1) box (FP32 check)
   int rad_malla;             // 1: RAD tables are exactly MALLA's recurrence (rad1 + (i-1)*radc)
   int merge_narrow;          // 1: narrow deposits may merge lanes (every non-centre weight product
-                             // <= 0.45, so four terms of a mass below 2^31 sum below 2^32)
+                             // <= 0.45, so eight terms of a mass below 2^30 sum below 2^32)
   int home_split;            // TSC count pass of the hierarchy binning: a member's entry in its
                              // home tile t counts under key 2t, its other entries under 2t + 1
   float rad1[3];

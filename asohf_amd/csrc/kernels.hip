@@ -522,13 +522,14 @@ __global__ void gate_kernel(const unsigned long long* tot_entries, const unsigne
 }
 
 // Fixed-point scale of the deterministic reduction: the power of two with
-// max(m) * scale in [2^30, 2^31), so every quantised term fits an int32 and a
-// sum over <= 2^32 particles an int64.
+// max(m) * scale in [2^29, 2^30), so every quantised term is below 2^30,
+// eight non-centre terms (each below half of it) sum below 2^32, and a sum
+// over <= 2^32 particles fits an int64.  The unit is at most max(m) * 2^-29.
 __device__ __forceinline__ double fixed_scale(unsigned mmax_bits) {
   const float mm = __uint_as_float(mmax_bits);
   int e = 0;
   if (mm > 0.f) (void)frexpf(mm, &e);
-  return ldexp(1.0, 31 - e);
+  return ldexp(1.0, 30 - e);
 }
 
 // ---------------------------------------------------------------------------
@@ -551,6 +552,15 @@ __device__ __forceinline__ long long glb_load(long long* a) {
 }
 __device__ __forceinline__ float glb_load(float* a) {
   return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A deterministic term: ((m*scale*wx)*wy)*wz rounded to the nearest integer
+// (halves up) as one fused multiply-add of 1/2 and a truncating conversion.
+// The terms are >= 0 and below 2^30; a term of 2^23 or more comes out within
+// one unit of the exact product's rounding.  Every term is a function of its
+// particle alone, so the sums stay independent of the order of the adds.
+__device__ __forceinline__ unsigned term_u32(float mxy, float wz) {
+  return (unsigned)__fmaf_rn(mxy, wz, 0.5f);
 }
 
 // 27-cell deposit of one particle into the LDS tile, which carries a 2-cell
@@ -583,7 +593,7 @@ __device__ __forceinline__ void deposit27(typename AccT<RED>::T* base, float m, 
           lds_add(reinterpret_cast<float*>(base) + (a + HX * (b + HY * c)), mxy[b][a] * wz[c]);
   } else {
     // every term is >= 0 (masses >= 0, TSC and kernel weights >= 0) and
-    // below 2^31, so it converts to an unsigned word whose 64-bit form needs
+    // below 2^30, so it converts to an unsigned word whose 64-bit form needs
     // no sign extension
     unsigned rest = 0;
 #pragma unroll
@@ -593,7 +603,7 @@ __device__ __forceinline__ void deposit27(typename AccT<RED>::T* base, float m, 
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           if (a == 1 && b == 1 && c == 1) continue;
-          const unsigned q = __float2uint_rn(mxy[b][a] * wz[c]);
+          const unsigned q = term_u32(mxy[b][a], wz[c]);
           rest += q;
           lds_add(reinterpret_cast<long long*>(base) + (a + HX * (b + HY * c)), (long long)(unsigned long long)q);
         }
@@ -653,11 +663,11 @@ constexpr int kMergeMin = 4;
 // Three butterfly steps: lane l takes lane l^1's terms when both stencils
 // start at the same cell (l even), then lane l^2's (bit 1 of l clear), then
 // lane l^4's (bit 2 clear).  Every non-centre TSC term is below half of
-// Q(m) < 2^31 (one factor is an outer weight: at most 0.5 * 0.75 * 0.75 with
-// exact centres, below 1/2 with the centre rounding the plan admits; table
-// mode merges only when the plan found every non-centre product of its row-0
-// weights at most 0.45), so four of them sum below 2^32 and the third step
-// adds in 64 bits; the centre remainder, negative for a light enough
+// Q(m) < 2^30 (one factor is an outer weight: at most 0.5 * 0.75 * 0.75 with
+// exact centres, below 0.48 with the centre rounding the plan admits;
+// table mode merges only when the plan found every non-centre product of its
+// row-0 weights at most 0.45), so eight of them sum below 2^32 and all three
+// steps add in 32 bits; the centre remainder, negative for a light enough
 // particle, merges as int64.
 // Every lane of the wave must be active (the steps read other lanes); ok =
 // the lane holds an entry, corner = its stencil's first cell in the tile.
@@ -726,13 +736,12 @@ __device__ __forceinline__ void deposit27_merged(typename AccT<RED>::T* acc, int
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         if (a == 1 && b == 1 && c == 1) continue;
-        unsigned q = __float2uint_rn(mxy[b][a] * wz[c]);
+        unsigned q = term_u32(mxy[b][a], wz[c]);
         rest += q;
         q += from_xor1(send1 ? q : 0u);
         if (step2) q += from_xor2(send2 ? q : 0u);
-        unsigned long long q3 = q;
-        if (step3) q3 += from_plus4(send3 ? q : 0u);
-        if (alive) lds_add(base + (a + HX * (b + HY * c)), (long long)q3);
+        if (step3) q += from_plus4(send3 ? q : 0u);
+        if (alive) lds_add(base + (a + HX * (b + HY * c)), (long long)(unsigned long long)q);
       }
   long long cen = (long long)__float2uint_rn(ms) - (long long)rest;
   cen += from_xor1(send1 ? cen : 0ll);

@@ -543,9 +543,9 @@ def accuracy(args, bld, snap, xyzm) -> dict:
     d1, d11, dgm = O.density64(sn, mode, nthreads=0)
     d1 = d1.reshape(-1)
     # the deterministic deposit's quantum: a fixed-point unit of at most
-    # max(m) * 2^-30 per term, in the output units of each level
+    # max(m) * 2^-29 per term, in the output units of each level
     mmax = float(sn.m.max())
-    per = np.array([mmax * 2.0 ** -30 / (float(g.denbas(int(L))) * float(g.fdm)) for L in range(g.nl + 1)])
+    per = np.array([mmax * 2.0 ** -29 / (float(g.denbas(int(L))) * float(g.fdm)) for L in range(g.nl + 1)])
     q1 = per[0]
     q11 = np.repeat(per[g.level_of_patch()], g.patch_cells())
     off = g.patch_offsets()
@@ -566,7 +566,7 @@ def accuracy(args, bld, snap, xyzm) -> dict:
            "target": "max rel err <= 1e-5 on cells above the mean; base mass to 1e-12",
            "note": "the sample thins the density 1:N/k, so many cells above the field mean hold one small "
                    "corner term; the deterministic deposit rounds every term to a fixed-point unit (at most "
-                   "max(m) 2^-30), so its figure is also given on cells of at least 2^21 units (2^-9 of a "
+                   "max(m) 2^-29), so its figure is also given on cells of at least 2^21 units (2^-8 of a "
                    "particle's mass); the atomic (FP32) deposit has no quantum"}
     for red in ("deterministic", "atomic"):
         r = bld.density(g, *dv, sn.npart_esp, mode=args.mode, reduction=red)
@@ -586,7 +586,7 @@ def accuracy(args, bld, snap, xyzm) -> dict:
         gm_rel = np.abs(gm - dgm)[nz] / np.abs(dgm[nz]) if nz.any() else np.zeros(1)
         # grids holding at least 2^21 fixed-point units of mass: below that a
         # grid reached by a few corner terms carries the quantum, not an error
-        big = dgm >= 2.0 ** 21 * mmax * 2.0 ** -30
+        big = dgm >= 2.0 ** 21 * mmax * 2.0 ** -29
         gm_rel_q = np.abs(gm - dgm)[big] / dgm[big] if big.any() else np.zeros(1)
         res[red] = {"fields": {"base": per_level["0"], "patches_all": errs(u11, d11)},
                     "fields_2p21_units": {"base": per_level_q["0"],

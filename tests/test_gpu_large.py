@@ -55,10 +55,10 @@ def host_sample(xyzm, esp_full, geom, k, seed):
 
 
 def cell_unit(geom, mmax):
-    """Output value of one fixed-point unit (2^-31 of the largest mass, the
-    deterministic deposit's quantum at most) per cell: u1, then u11 cells."""
-    u0 = mmax * 2.0 ** -30 / (float(geom.denbas(0)) * float(geom.fdm))
-    per = np.array([mmax * 2.0 ** -30 / (float(geom.denbas(int(l))) * float(geom.fdm))
+    """Output value of one fixed-point unit (at most 2^-29 of the largest mass,
+    the deterministic deposit's quantum) per cell: u1, then u11 cells."""
+    u0 = mmax * 2.0 ** -29 / (float(geom.denbas(0)) * float(geom.fdm))
+    per = np.array([mmax * 2.0 ** -29 / (float(geom.denbas(int(l))) * float(geom.fdm))
                     for l in range(geom.nl + 1)])
     return u0, np.repeat(per[geom.level_of_patch()], geom.patch_cells())
 
@@ -79,8 +79,8 @@ def sample_parity(bld, snap, xyzm, reduction, seed, tol):
     FP64 oracle: fields and per-grid masses.  The sample thins the density by
     the sampling factor, so many cells above the field's mean hold a single
     small corner term.  The deterministic deposit rounds every term to a
-    fixed-point unit (at most max(m) * 2^-30), so it is held to the bar on
-    cells of at least 2^21 units (2^-9 of a particle's mass; a full build's
+    fixed-point unit (at most max(m) * 2^-29), so it is held to the bar on
+    cells of at least 2^21 units (2^-8 of a particle's mass; a full build's
     cells above the mean hold about a particle's mass or more); the
     FP32-atomic deposit, whose terms are not quantised, on every cell above
     the mean."""
@@ -107,7 +107,7 @@ def sample_parity(bld, snap, xyzm, reduction, seed, tol):
     assert np.all((dgm > 0) | (gm == 0))
     # per-grid FP64 masses: a grid's deposit is the sum of its members'
     # terms, computed in FP32 as the reference does (a few ulps each) and,
-    # deterministic, rounded to the fixed-point unit (at most max(m) * 2^-30,
+    # deterministic, rounded to the fixed-point unit (at most max(m) * 2^-29,
     # half of it per term, 27 terms per member); a member whose whole stencil
     # lies in the grid deposits exactly its quantised mass, so only members
     # clipped at a patch border carry the FP32 error; atomic terms add FP32
@@ -116,7 +116,7 @@ def sample_parity(bld, snap, xyzm, reduction, seed, tol):
                        .cpu().numpy()[:, 0], minlength=snap.geom.npatch_total + 1)
     mmax = float(hs.m.max())
     if reduction == "deterministic":
-        tol = 1e-6 * np.abs(dgm) + 13.5 * mmax * 2.0 ** -30 * memb
+        tol = 1e-6 * np.abs(dgm) + 13.5 * mmax * 2.0 ** -29 * memb
         tot = float(hs.m.astype(np.float64).sum())
         assert abs(gm[0] - tot) <= 1e-12 * tot     # the base grid holds every stencil whole
     else:

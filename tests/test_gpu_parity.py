@@ -139,7 +139,7 @@ def test_clustered_stencils_merge_exactly(bld):
     at a tile halo's first cell (centre in the cell just below the tile), so the
     deposit sums lanes in registers in all its merge steps.  Masses span 1 to
     1e-7, so some centre remainders are negative.  Every particle deposits
-    exactly its quantised mass Q(m) = rn(m * scale) (scale = 2^(31-e), e the
+    exactly its quantised mass Q(m) = rn(m * scale) (scale = 2^(30-e), e the
     binary exponent of the largest mass), so the base-grid mass equals
     sum(Q) / scale exactly; the fields are bit-identical under a shuffled
     particle order and across tile shapes, and they match the FP64 oracle."""
@@ -163,7 +163,7 @@ def test_clustered_stencils_merge_exactly(bld):
     snap = synth.Snapshot(pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy(), m, esp, g)
     print(f"case clustered stencils: {n} particles", flush=True)
     ref = run(bld, snap, "tsc")
-    scale = 2.0 ** (31 - int(np.frexp(np.float32(m.max()))[1]))
+    scale = 2.0 ** (30 - int(np.frexp(np.float32(m.max()))[1]))
     q = np.rint(m.astype(np.float64) * scale)   # m * scale is exact; rint rounds half to even
     assert ref[2][0] == q.sum() / scale
     tot = float(m.astype(np.float64).sum())

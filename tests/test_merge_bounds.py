@@ -1,10 +1,10 @@
 """Bounds the lane merging of the 27-cell deposit relies on.
 
-The deterministic deposit sums the rounded terms of up to four lanes in 32
-bits before its third butterfly step (which adds in 64 bits).  That is exact
-only if every non-centre term stays below 2^32 / 4.  A term is
-((m * scale) * Kx) * Ky * Kz with m * scale < 2^31, so the bound is on the
-weight products:
+The deterministic deposit sums the rounded terms of up to eight lanes in 32
+bits (three butterfly steps).  That is exact only if every non-centre term
+stays below 2^32 / 8.  A term is ((m * scale) * Kx) * Ky * Kz with
+m * scale < 2^30, so the bound is on the weight products (below 1/2, with
+the rounding of each term to an integer):
 
   * TSC: the weights of a particle at offset t (in cells) from its centre cell
     are 0.5 (0.5 - t)^2, 0.75 - t^2, 0.5 (0.5 + t)^2 (grids.f:1194-1213),
@@ -82,13 +82,16 @@ def test_tsc_weight_products_fit_the_merge():
     b = np.stack([np.abs(1 + tt + dd[0]), np.abs(tt + dd[1]), np.abs(1 - tt + dd[2])]).astype(f32)
     wc = np.stack([f32(0.5) * (f32(1.5) - b[0]) ** 2, f32(0.75) - b[1] * b[1], f32(0.5) * (f32(1.5) - b[2]) ** 2])
     assert np.all(wc >= 0)
-    # below 1/2: four terms of a mass below 2^31 still sum below 2^32
-    assert max_non_centre_product(wc.max(axis=1)) < 0.5
-    # four terms of a mass below 2^31 fit 32 bits; eight do not (hence the
-    # third step's 64-bit add)
-    q_max = np.floor(bound * 2.0 ** 31) + 1
-    assert 4 * q_max < 2 ** 32
-    assert 8 * q_max >= 2 ** 32
+    # below 1/2: eight terms of a mass below 2^30 still sum below 2^32
+    dev = max_non_centre_product(wc.max(axis=1))
+    assert dev < 0.48
+    for b_ in (bound, dev):
+        q_max = np.floor(b_ * 2.0 ** 30) + 1   # a term rounded to the nearest integer
+        assert 8 * q_max < 2 ** 32
+    # the analytic worst case with the centre rounding: three outer weights of
+    # a centre a quarter cell off, at the end of the accepted range
+    w_out = 0.5 * (1.25 + 2.0 ** -10) ** 2
+    assert dev <= w_out ** 3 < 0.48
 
 
 def test_table_row0_products_pass_the_plan_check():
