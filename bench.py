@@ -430,11 +430,11 @@ def side_routines(args, bld, spec, snap, g, xyzm, kw, n, n_cells, out):
             # the density build on the hierarchy CREATE_MESH made for these
             # particles, both modes (deterministic)
             for md in ("tsc", "table"):
-                bld.density(mg, x, y, z, snap.npart_esp, mode=md)
+                bld.density(mg, x, y, z, m, snap.npart_esp, mode=md)
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 for _ in range(args.steps):
-                    bld.density(mg, x, y, z, snap.npart_esp, mode=md)
+                    bld.density(mg, x, y, z, m, snap.npart_esp, mode=md)
                 torch.cuda.synchronize()
                 tb = (time.perf_counter() - t0) / args.steps
                 st_m = bld.stats()

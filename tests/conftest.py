@@ -9,7 +9,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-FIXTURES = ("seeded_11", "seeded_12", "seeded_13_cic", "edges", "seeded_14_fdm")
+FIXTURES = ("seeded_11", "seeded_12", "seeded_13_cic", "edges", "seeded_14_fdm", "seeded_15_128")
 
 
 def pytest_configure(config):

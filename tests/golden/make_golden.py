@@ -17,7 +17,9 @@ Cases:
                 either side), and on fine-cell faces inside patches, with every
                 species present so every kernel width occurs;
   seeded_14_fdm -- 64,000 particles on a 64^3 base grid with four AMR
-                levels, divided by the reference's default FDM = 0.845.
+                levels, divided by the reference's default FDM = 0.845;
+  seeded_15_128 -- 262,144 particles on a 128^3 base grid (the reference
+                build's NMAX), four AMR levels, about 60 patches, FDM 0.845.
 
 Usage: python tests/golden/make_golden.py [names]   (needs `make -C oracle ref`)
 """
@@ -156,6 +158,16 @@ def main() -> None:
         snap = synth.make_snapshot(synth.WORKLOADS[1], n_side=40, nx=64, patches_per_level=(14, 10, 8, 5),
                                    side_hi=32, seed=14, fdm=0.845)
         save("seeded_14_fdm", snap)
+    # the planned size of the reference's parity fixtures: 64^3 particles on a
+    # 128^3 base grid (the reference build's NMAX), about 60 patches
+    # (98% of the particles in haloes, so most base cells stay empty and the
+    # fixture compresses to a few MB)
+    if want("seeded_15_128"):
+        import dataclasses
+        spec = dataclasses.replace(synth.WORKLOADS[1], f_halo=0.98)
+        snap = synth.make_snapshot(spec, n_side=64, nx=128, patches_per_level=(20, 16, 14, 10),
+                                   side_hi=32, seed=15, fdm=0.845)
+        save("seeded_15_128", snap)
 
 
 if __name__ == "__main__":
