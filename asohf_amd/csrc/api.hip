@@ -1072,7 +1072,7 @@ static asohf_status enqueue_hier(asohf_ctx* ctx, const PipeArgs& a, hipStream_t 
   HIPCHK(launch_fill_u32(&a.C->n_items_eff, (unsigned)a.n_items, 1, s), "queue length");
   HIPCHK(launch_fill_u32(a.item_pairs, 0u, a.n_items, s), "clearing item sums");
   HIPCHK(launch_cursor_reset(a.start, a.cursor, a.nkeys, a.cus, s), "cursors");
-  HIPCHK(launch_bin_base_tsc(a.P, a.A, a.cursor, a.start, const_cast<int4*>(a.D.entries), a.C, a.BT, s),
+  HIPCHK(launch_bin_base_tsc(a.P, a.A, a.cursor, a.start, const_cast<int4*>(a.D.entries), a.C, a.BT, a.cus, s),
          "base-grid binning");
   if (a.A.n > 0) HIPCHK(launch_reduce_blocks(a.BT, hier_bin_blocks(a.A.n), a.C, s), "block sums");
   if (events) {

@@ -192,14 +192,16 @@ __device__ __forceinline__ void reserve_round(const int* keys, unsigned* vals, u
   }
 }
 
-// Base-grid binning: kHbPer particles per thread.  Each valid particle
+// Base-grid binning, persistent: each workgroup takes chunks of kHbChunk
+// particles (kHbPer per thread) in turn and loads its next chunk while it
+// bins this one.  Each valid particle
 // (finite, mass >= 0, within the TSC range of the base grid) emits one entry
 // per base tile its wrapped stencil touches.  Every wave lists its emissions
 // (tile key, particle) in its own LDS list, then counts the list 64 at a time
 // (a position in the key's range of the workgroup, written back over the
 // list word); one global add per key reserves the workgroup's ranges and
 // every wave writes its listed entries, reading each particle again.
-// Per-workgroup partials: base members and the largest mass.
+// Per-chunk partials: base members and the largest mass.
 __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsigned long long* cursor,
                                                            const unsigned long long* start, int4* entries,
                                                            Counters* C, BlockTables BT) {
@@ -211,105 +213,123 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
   __shared__ float s_mmax[kHbThreads / 64];
   const int tid = threadIdx.x, lane = (int)__lane_id();
   unsigned long long* wl = wlist_all + (tid >> 6) * kHbWList;
-  for (int k = tid; k < kHbCap; k += kHbThreads) { keys[k] = -1; vals[k] = 0u; }
-  __syncthreads();
   const GridDesc& B = P.grids[0];
   const int bn[3] = {B.n[0], B.n[1], B.n[2]};
   const float bd[3] = {B.d[0], B.d[1], B.d[2]};
   const float bid[3] = {B.inv_d[0], B.inv_d[1], B.inv_d[2]};
   const int bnt0 = B.nt[0], bnt1 = B.nt[1];
-  const long long c0 = (long long)blockIdx.x * kHbChunk;
   auto particle = [&](long long i) {
     return make_int4(__float_as_int(A.x[i]), __float_as_int(A.y[i]), __float_as_int(A.z[i]), __float_as_int(A.m[i]));
   };
-  unsigned err = 0;
-  unsigned long long valid = 0;
-  float mmax = 0.f;
-  int nw = 0;   // the wave's list length (uniform)
+  const long long nchunks = (A.n + kHbChunk - 1) / kHbChunk;
+  // the workgroup's next chunk of particles, loaded while this one is binned
+  float fx[kHbPer], fy[kHbPer], fz[kHbPer], fm[kHbPer];
+  auto fetch = [&](long long cc) {
 #pragma unroll
-  for (int r = 0; r < kHbPer; ++r) {
-    const int k = r * kHbThreads + tid;
-    bool ok = c0 + k < A.n;
-    float x = 0.f, y = 0.f, z = 0.f, m = 0.f;
-    if (ok) { x = A.x[c0 + k]; y = A.y[c0 + k]; z = A.z[c0 + k]; m = A.m[c0 + k]; }
-    if (ok && !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) { err |= kErrNonFinite; ok = false; }
-    if (ok && m < 0.f) { err |= kErrNegMass; ok = false; }
-    if (ok && !(tsc_base_in_range(x, P.xl, bid[0], bn[0]) && tsc_base_in_range(y, P.xl, bid[1], bn[1]) &&
-                tsc_base_in_range(z, P.xl, bid[2], bn[2]))) {
-      err |= kErrWrap;
-      ok = false;
+    for (int r = 0; r < kHbPer; ++r) {
+      const long long i = cc * kHbChunk + r * kHbThreads + tid;
+      fx[r] = fy[r] = fz[r] = fm[r] = 0.f;
+      if (cc < nchunks && i < A.n) { fx[r] = A.x[i]; fy[r] = A.y[i]; fz[r] = A.z[i]; fm[r] = A.m[i]; }
     }
-    int tx[3] = {0, 0, 0}, ty[3] = {0, 0, 0}, tz[3] = {0, 0, 0};
-    int nt = 0, nx_ = 1, ny_ = 1;
-    int hb[3] = {0, 0, 0};   // the home tile
-    if (ok) {
-      ++valid;
-      mmax = fmaxf(mmax, m);
-      nx_ = base_axis_tiles(tsc_base_index(x, P.xl, bd[0], bid[0], bn[0]), bn[0], P.tshift[0], tx);
-      ny_ = base_axis_tiles(tsc_base_index(y, P.xl, bd[1], bid[1], bn[1]), bn[1], P.tshift[1], ty);
-      const int nz_ = base_axis_tiles(tsc_base_index(z, P.xl, bd[2], bid[2], bn[2]), bn[2], P.tshift[2], tz);
-      nt = nx_ * ny_ * nz_;
-      hb[0] = (home_cell(x, P.xl, bid[0], true, bn[0]) - 1) >> P.tshift[0];
-      hb[1] = (home_cell(y, P.xl, bid[1], true, bn[1]) - 1) >> P.tshift[1];
-      hb[2] = (home_cell(z, P.xl, bid[2], true, bn[2]) - 1) >> P.tshift[2];
+  };
+  fetch(blockIdx.x);
+  for (long long chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const long long c0 = chunk * kHbChunk;
+    float px[kHbPer], py[kHbPer], pz[kHbPer], pm[kHbPer];
+#pragma unroll
+    for (int r = 0; r < kHbPer; ++r) { px[r] = fx[r]; py[r] = fy[r]; pz[r] = fz[r]; pm[r] = fm[r]; }
+    fetch(chunk + gridDim.x);
+    for (int k = tid; k < kHbCap; k += kHbThreads) { keys[k] = -1; vals[k] = 0u; }
+    __syncthreads();
+    unsigned err = 0;
+    unsigned long long valid = 0;
+    float mmax = 0.f;
+    int nw = 0;   // the wave's list length (uniform)
+#pragma unroll
+    for (int r = 0; r < kHbPer; ++r) {
+      const int k = r * kHbThreads + tid;
+      bool ok = c0 + k < A.n;
+      const float x = px[r], y = py[r], z = pz[r], m = pm[r];
+      if (ok && !(isfinite(x) && isfinite(y) && isfinite(z) && isfinite(m))) { err |= kErrNonFinite; ok = false; }
+      if (ok && m < 0.f) { err |= kErrNegMass; ok = false; }
+      if (ok && !(tsc_base_in_range(x, P.xl, bid[0], bn[0]) && tsc_base_in_range(y, P.xl, bid[1], bn[1]) &&
+                  tsc_base_in_range(z, P.xl, bid[2], bn[2]))) {
+        err |= kErrWrap;
+        ok = false;
+      }
+      int tx[3] = {0, 0, 0}, ty[3] = {0, 0, 0}, tz[3] = {0, 0, 0};
+      int nt = 0, nx_ = 1, ny_ = 1;
+      int hb[3] = {0, 0, 0};   // the home tile
+      if (ok) {
+        ++valid;
+        mmax = fmaxf(mmax, m);
+        nx_ = base_axis_tiles(tsc_base_index(x, P.xl, bd[0], bid[0], bn[0]), bn[0], P.tshift[0], tx);
+        ny_ = base_axis_tiles(tsc_base_index(y, P.xl, bd[1], bid[1], bn[1]), bn[1], P.tshift[1], ty);
+        const int nz_ = base_axis_tiles(tsc_base_index(z, P.xl, bd[2], bid[2], bn[2]), bn[2], P.tshift[2], tz);
+        nt = nx_ * ny_ * nz_;
+        hb[0] = (home_cell(x, P.xl, bid[0], true, bn[0]) - 1) >> P.tshift[0];
+        hb[1] = (home_cell(y, P.xl, bid[1], true, bn[1]) - 1) >> P.tshift[1];
+        hb[2] = (home_cell(z, P.xl, bid[2], true, bn[2]) - 1) >> P.tshift[2];
+      }
+      int tot;
+      const int off = nw + wave_prefix5(nt, tot);
+      for (int c = 0; c < nt; ++c) {
+        const int q = div_small(c, nx_), qz = div_small(q, ny_);
+        const int a = c - q * nx_, b = q - qz * ny_;
+        const int ta = a == 0 ? tx[0] : (a == 1 ? tx[1] : tx[2]);
+        const int tb = b == 0 ? ty[0] : (b == 1 ? ty[1] : ty[2]);
+        const int tcz = qz == 0 ? tz[0] : (qz == 1 ? tz[1] : tz[2]);
+        const int key = 2 * ((tcz * bnt1 + tb) * bnt0 + ta) + ((ta == hb[0] && tb == hb[1] && tcz == hb[2]) ? 0 : 1);
+        if (off + c < kHbWList) wl[off + c] = (unsigned long long)(unsigned)key | ((unsigned long long)k << 32);
+        else emit_direct(cursor, start, entries, C, key, particle(c0 + k));
+      }
+      nw = min(nw + tot, kHbWList);
     }
-    int tot;
-    const int off = nw + wave_prefix5(nt, tot);
-    for (int c = 0; c < nt; ++c) {
-      const int q = div_small(c, nx_), qz = div_small(q, ny_);
-      const int a = c - q * nx_, b = q - qz * ny_;
-      const int ta = a == 0 ? tx[0] : (a == 1 ? tx[1] : tx[2]);
-      const int tb = b == 0 ? ty[0] : (b == 1 ? ty[1] : ty[2]);
-      const int tcz = qz == 0 ? tz[0] : (qz == 1 ? tz[1] : tz[2]);
-      const int key = 2 * ((tcz * bnt1 + tb) * bnt0 + ta) + ((ta == hb[0] && tb == hb[1] && tcz == hb[2]) ? 0 : 1);
-      if (off + c < kHbWList) wl[off + c] = (unsigned long long)(unsigned)key | ((unsigned long long)k << 32);
-      else emit_direct(cursor, start, entries, C, key, particle(c0 + k));
-    }
-    nw = min(nw + tot, kHbWList);
-  }
-  // count the list: a position in the key's range of the workgroup
-  for (int b0 = 0; b0 < nw; b0 += 64) {
-    const int idx = b0 + lane;
-    const bool want = idx < nw;
-    const unsigned long long w = want ? wl[idx] : 0ull;
-    int h;
-    unsigned at;
-    count_round<kHbCap>(keys, vals, want, (int)(unsigned)w, h, at);
-    if (want) {
-      if (h >= 0) {
-        wl[idx] = (w & 0xffffffff00000000ull) | ((unsigned long long)h << 16) | at;
-      } else {
-        wl[idx] = kEmSkip;
-        emit_direct(cursor, start, entries, C, (int)(unsigned)w, particle(c0 + (long long)(w >> 32)));
+    // count the list: a position in the key's range of the workgroup
+    for (int b0 = 0; b0 < nw; b0 += 64) {
+      const int idx = b0 + lane;
+      const bool want = idx < nw;
+      const unsigned long long w = want ? wl[idx] : 0ull;
+      int h;
+      unsigned at;
+      count_round<kHbCap>(keys, vals, want, (int)(unsigned)w, h, at);
+      if (want) {
+        if (h >= 0) {
+          wl[idx] = (w & 0xffffffff00000000ull) | ((unsigned long long)h << 16) | at;
+        } else {
+          wl[idx] = kEmSkip;
+          emit_direct(cursor, start, entries, C, (int)(unsigned)w, particle(c0 + (long long)(w >> 32)));
+        }
       }
     }
-  }
-  __syncthreads();
-  reserve_round<kHbCap, kHbThreads>(keys, vals, base, cursor, start, C);
-  __syncthreads();
-  for (int idx = lane; idx < nw; idx += 64) {
-    const unsigned long long w = wl[idx];
-    if (w == kEmSkip) continue;
-    const int h = (int)((w >> 16) & 0xffffu);
-    const unsigned long long pos = base[h] + (w & 0xffffu);
-    if (pos < start[keys[h] + 1]) entries[pos] = particle(c0 + (long long)(w >> 32));
-  }
-  // per-workgroup partials (members of the base grid, largest mass)
-  valid = wave_sum(valid);
-  mmax = wave_max(mmax);
-  err = wave_or(err);
-  if ((tid & 63) == 0) {
-    s_valid[tid >> 6] = valid;
-    s_mmax[tid >> 6] = mmax;
-    if (err) atomicOr(&C->err, err);
-  }
-  __syncthreads();
-  if (tid == 0) {
-    unsigned long long p = 0;
-    float mm = 0.f;
-    for (int w = 0; w < kHbThreads / 64; ++w) { p += s_valid[w]; mm = fmaxf(mm, s_mmax[w]); }
-    BT.pairs[blockIdx.x] = p;
-    BT.mmax[blockIdx.x] = __float_as_uint(mm);
+    __syncthreads();
+    reserve_round<kHbCap, kHbThreads>(keys, vals, base, cursor, start, C);
+    __syncthreads();
+    for (int idx = lane; idx < nw; idx += 64) {
+      const unsigned long long w = wl[idx];
+      if (w == kEmSkip) continue;
+      const int h = (int)((w >> 16) & 0xffffu);
+      const unsigned long long pos = base[h] + (w & 0xffffu);
+      if (pos < start[keys[h] + 1]) entries[pos] = particle(c0 + (long long)(w >> 32));
+    }
+    // per-workgroup partials (members of the base grid, largest mass)
+    valid = wave_sum(valid);
+    mmax = wave_max(mmax);
+    err = wave_or(err);
+    if ((tid & 63) == 0) {
+      s_valid[tid >> 6] = valid;
+      s_mmax[tid >> 6] = mmax;
+      if (err) atomicOr(&C->err, err);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      unsigned long long p = 0;
+      float mm = 0.f;
+      for (int w = 0; w < kHbThreads / 64; ++w) { p += s_valid[w]; mm = fmaxf(mm, s_mmax[w]); }
+      BT.pairs[chunk] = p;
+      BT.mmax[chunk] = __float_as_uint(mm);
+    }
+    __syncthreads();   // the chunk's writes read keys and base; its partials s_valid
   }
 }
 
@@ -517,10 +537,18 @@ hipError_t launch_count_to_capacity(unsigned* cnt, long long nkeys, int cus, hip
 
 hipError_t launch_bin_base_tsc(const Plan& P, const Parts& A, unsigned long long* cursor,
                                const unsigned long long* start, int4* entries, Counters* C,
-                               const BlockTables& BT, hipStream_t s) {
+                               const BlockTables& BT, int cus, hipStream_t s) {
   const long long nb = hier_bin_blocks(A.n);
   if (nb == 0) return hipSuccess;
-  bin_base_tsc<<<(unsigned)nb, kHbThreads, 0, s>>>(P, A, cursor, start, entries, C, BT);
+  static int per_cu = 0;
+  if (per_cu <= 0) {
+    int k = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&k, bin_base_tsc, kHbThreads, 0) != hipSuccess || k <= 0) k = 4;
+    per_cu = k;
+  }
+  // the workgroups that fit the chip at once, each taking chunks in turn
+  const long long grid = std::min<long long>(nb, (long long)cus * per_cu);
+  bin_base_tsc<<<(unsigned)grid, kHbThreads, 0, s>>>(P, A, cursor, start, entries, C, BT);
   return hipGetLastError();
 }
 

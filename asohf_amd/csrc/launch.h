@@ -44,7 +44,7 @@ hipError_t launch_cursor_reset(const unsigned long long* start, unsigned long lo
 hipError_t launch_count_to_capacity(unsigned* cnt, long long nkeys, int cus, hipStream_t s);
 hipError_t launch_bin_base_tsc(const Plan& P, const Parts& A, unsigned long long* cursor,
                                const unsigned long long* start, int4* entries, Counters* C,
-                               const BlockTables& BT, hipStream_t s);
+                               const BlockTables& BT, int cus, hipStream_t s);
 hipError_t launch_emit_tsc(const Plan& P, const Item* items, long long item_lo, long long item_hi, int level,
                            unsigned long long* cursor, const unsigned long long* start, int4* entries,
                            unsigned* item_pairs, Counters* C, int cus, hipStream_t s);
