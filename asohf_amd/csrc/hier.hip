@@ -338,7 +338,8 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
 // `batch` consecutive items at a time from the level's queue counter and
 // works through those whose tile has targets (the others' member counts stay
 // 0).  Per item the target records are staged in LDS (kEmTargets at a time);
-// the item's entries are taken kEmRound per round, kEmPer per thread.  In a
+// the item's entries are taken kEmPer per thread per round (one per round
+// where overlapping patches make the lists long).  In a
 // round every wave tests its entries against the targets and lists its
 // member emissions (destination tile key, entry) in its own LDS list, then
 // counts the list 64 emissions at a time (a position in the key's range of
@@ -358,8 +359,13 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
   __shared__ BoxRec recs[kEmTargets];
   __shared__ long long s_i0;
   __shared__ unsigned s_red[kEmThreads / 64];
+  __shared__ int s_nwmax[2];   // the waves' longest list of a round, by round parity
   const int tid = threadIdx.x, lane = (int)__lane_id();
   unsigned long long* wl = wlist_all + (tid >> 6) * kEmWList;
+  // wave-steps per round: kEmPer, or 1 after a round whose lists would have
+  // overflowed with kEmPer (members of overlapping patches emit several
+  // entries each); kept from one item to the next
+  int steps = kEmPer;
   unsigned* ml = mlist_all + (tid >> 6) * 128;
   unsigned* queue = &C->equeue[level * kLine];
   for (;;) {
@@ -381,17 +387,21 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
       const long long e1 = min(min(it.end, (long long)start[2 * t + 1]), (long long)cursor[2 * t]);
       if (e1 <= e0) continue;
       for (int k = tid; k < kEmCap; k += kEmThreads) { keys[k] = -1; vals[k] = 0u; }
+      if (tid < 2) s_nwmax[tid] = 0;
+      int par = 0;
       unsigned pairs = 0;
       for (unsigned q = q0; q < q1; q += kEmTargets) {
         const int nq = (int)min((unsigned)kEmTargets, q1 - q);
         for (int k = tid; k < nq * 8; k += kEmThreads)
           reinterpret_cast<int4*>(recs)[k] = reinterpret_cast<const int4*>(P.grec + P.tgt[q + (k >> 3)])[k & 7];
         __syncthreads();
-        for (long long r0 = e0; r0 < e1; r0 += kEmRound) {
+        for (long long r0 = e0, rn = 0; r0 < e1; r0 += rn) {
+          const int nsteps = steps;
+          rn = (long long)nsteps * kEmThreads;
           // ---- list the wave's member emissions
           int nw = 0;   // the wave's list length (uniform)
 #pragma unroll 1
-          for (int i = 0; i < kEmPer; ++i) {
+          for (int i = 0; i < nsteps; ++i) {
             const int k = i * kEmThreads + tid;
             const bool ok = r0 + k < e1;
             const int4 v = ok ? entries[r0 + k] : make_int4(0, 0, 0, 0);
@@ -470,7 +480,14 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
               }
             }
           }
+          if (lane == 0) atomicMax(&s_nwmax[par], nw);
+          if (tid == 0) s_nwmax[par ^ 1] = 0;   // the next round's, before this barrier
           __syncthreads();
+          {
+            const int mx = s_nwmax[par];
+            steps = (mx * kEmPer > nsteps * (kEmWList * 3 / 4)) ? 1 : kEmPer;
+            par ^= 1;
+          }
           reserve_round<kEmCap, kEmThreads>(keys, vals, base, cursor, start, C);
           __syncthreads();
           // ---- write the listed entries
