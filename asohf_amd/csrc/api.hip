@@ -1030,6 +1030,7 @@ static asohf_status hier_plan(asohf_ctx* ctx, PipeArgs& a, hipStream_t s) {
   const unsigned long long nsplit = ctx->hscalar[1];
   const long long n_items = (long long)ctx->hscalar[2 + nl + 1];
   if (n_items >= (1ll << 31)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^31 work items");
+  if (cap_entries >= (1ull << 40)) return fail(ctx, ASOHF_ERR_CAPACITY, "more than 2^40 tile entries");
   for (int l = 0; l <= nl + 1; ++l) ctx->hier_items[l] = (long long)ctx->hscalar[2 + l];
   const int T = a.P.tile[0] * a.P.tile[1] * a.P.tile[2];
   HIPCHK(ctx->entries.ensure(sizeof(int4) * std::max<unsigned long long>(cap_entries, 1)), "allocating entries");

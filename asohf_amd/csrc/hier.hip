@@ -175,7 +175,14 @@ __device__ __forceinline__ int wave_prefix(int v, int& total) {
 __device__ __forceinline__ int wave_prefix4(int v, int& total) { return wave_prefix<4>(v, total); }
 __device__ __forceinline__ int wave_prefix5(int v, int& total) { return wave_prefix<5>(v, total); }
 
+// a reserved range in one word: its first slot (below 2^40; the host rejects
+// larger entry buffers) and, above it, how many of its slots lie within the
+// key's capacity (all of them unless the reservation overflowed)
+constexpr int kSlotBits = 40;
+constexpr unsigned long long kSlotMask = (1ull << kSlotBits) - 1ull;
+
 // one reservation per tile key of the round: base[k] = the key's first slot
+// and its room
 template <int CAP, int NTHR>
 __device__ __forceinline__ void reserve_round(const int* keys, unsigned* vals, unsigned long long* base,
                                               unsigned long long* cursor, const unsigned long long* start,
@@ -185,11 +192,18 @@ __device__ __forceinline__ void reserve_round(const int* keys, unsigned* vals, u
     const unsigned v = vals[k];
     if (key >= 0 && v) {
       const unsigned long long b0 = atomicAdd(&cursor[key], (unsigned long long)v);
-      base[k] = b0;
-      if (b0 + v > start[key + 1]) set_overflow(C);
+      const unsigned long long lim = start[key + 1];
+      const unsigned long long room = b0 >= lim ? 0ull : (lim - b0 < v ? lim - b0 : (unsigned long long)v);
+      base[k] = b0 | (room << kSlotBits);
+      if (b0 + v > lim) set_overflow(C);
     }
     vals[k] = 0u;
   }
+}
+
+// the slot of position `at` of a reserved range, or -1 past the key's capacity
+__device__ __forceinline__ long long reserved_slot(unsigned long long b, unsigned at) {
+  return at < (unsigned)(b >> kSlotBits) ? (long long)((b & kSlotMask) + at) : -1ll;
 }
 
 // Base-grid binning, persistent: each workgroup takes chunks of kHbChunk
@@ -308,9 +322,8 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
     for (int idx = lane; idx < nw; idx += 64) {
       const unsigned long long w = wl[idx];
       if (w == kEmSkip) continue;
-      const int h = (int)((w >> 16) & 0xffffu);
-      const unsigned long long pos = base[h] + (w & 0xffffu);
-      if (pos < start[keys[h] + 1]) entries[pos] = particle(c0 + (long long)(w >> 32));
+      const long long pos = reserved_slot(base[(w >> 16) & 0xffffu], (unsigned)(w & 0xffffu));
+      if (pos >= 0) entries[pos] = particle(c0 + (long long)(w >> 32));
     }
     // per-workgroup partials (members of the base grid, largest mass)
     valid = wave_sum(valid);
@@ -329,7 +342,7 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
       BT.pairs[chunk] = p;
       BT.mmax[chunk] = __float_as_uint(mm);
     }
-    __syncthreads();   // the chunk's writes read keys and base; its partials s_valid
+    __syncthreads();   // the chunk's writes read base; its partials s_valid
   }
 }
 
@@ -494,9 +507,8 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
           for (int idx = lane; idx < nw; idx += 64) {
             const unsigned long long w = wl[idx];
             if (w == kEmSkip) continue;
-            const int h = (int)((w >> 16) & 0xffffu);
-            const unsigned long long pos = base[h] + (w & 0xffffu);
-            if (pos < start[keys[h] + 1]) entries[pos] = entries[r0 + (long long)(w >> 32)];
+            const long long pos = reserved_slot(base[(w >> 16) & 0xffffu], (unsigned)(w & 0xffffu));
+            if (pos >= 0) entries[pos] = entries[r0 + (long long)(w >> 32)];
           }
         }
       }
