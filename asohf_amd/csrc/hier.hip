@@ -214,7 +214,8 @@ __device__ __forceinline__ long long reserved_slot(unsigned long long b, unsigne
 // (tile key, particle) in its own LDS list, then counts the list 64 at a time
 // (a position in the key's range of the workgroup, written back over the
 // list word); one global add per key reserves the workgroup's ranges and
-// every wave writes its listed entries, reading each particle again.
+// every wave writes its listed entries, taking each particle from the lane
+// that holds it.
 // Per-chunk partials: base members and the largest mass.
 __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsigned long long* cursor,
                                                            const unsigned long long* start, int4* entries,
@@ -319,11 +320,24 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
     __syncthreads();
     reserve_round<kHbCap, kHbThreads>(keys, vals, base, cursor, start, C);
     __syncthreads();
-    for (int idx = lane; idx < nw; idx += 64) {
-      const unsigned long long w = wl[idx];
+    // the listed particle comes from the lane of this wave that loaded it
+    // (k = r * kHbThreads + tid: lane k % 64, register slot k / kHbThreads);
+    // every lane of the wave takes part in the shuffles
+    for (int b0 = 0; b0 < nw; b0 += 64) {
+      const int idx = b0 + lane;
+      const unsigned long long w = idx < nw ? wl[idx] : kEmSkip;
+      const int k = (int)(unsigned)(w >> 32);
+      const int src = k & 63, slot = (int)((unsigned)k / (unsigned)kHbThreads);
+      int4 e = make_int4(0, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < kHbPer; ++r) {
+        const int4 u = make_int4(__shfl(__float_as_int(px[r]), src), __shfl(__float_as_int(py[r]), src),
+                                 __shfl(__float_as_int(pz[r]), src), __shfl(__float_as_int(pm[r]), src));
+        if (slot == r) e = u;
+      }
       if (w == kEmSkip) continue;
       const long long pos = reserved_slot(base[(w >> 16) & 0xffffu], (unsigned)(w & 0xffffu));
-      if (pos >= 0) entries[pos] = particle(c0 + (long long)(w >> 32));
+      if (pos >= 0) entries[pos] = e;
     }
     // per-workgroup partials (members of the base grid, largest mass)
     valid = wave_sum(valid);
