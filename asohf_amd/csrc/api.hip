@@ -890,7 +890,7 @@ static asohf_status enqueue_count(asohf_ctx* ctx, const PipeArgs& a, hipStream_t
   HIPCHK(launch_fill_u32((unsigned*)a.C, 0u, sizeof(Counters) / 4, s), "clearing counters");
   HIPCHK(launch_fill_u32(a.cnt, 0u, a.nkeys, s), "clearing counts");
   HIPCHK(launch_count(a.P, a.A, a.cnt, a.C, a.BT, a.cus, s), "count kernel");
-  if (a.A.n > 0) HIPCHK(launch_reduce_blocks(a.BT, bin_blocks(a.A.n), a.C, s), "block sums");
+  if (a.A.n > 0) HIPCHK(launch_reduce_blocks(a.BT, bin_blocks(a.A.n), a.C, a.cus, s), "block sums");
   HIPCHK(launch_scan(a.cnt, a.nkeys, a.bsum, a.start, a.cursor, s), "scan");
   HIPCHK(launch_plan_count(a.start, a.ntiles, a.chunk, a.chunk_wide, a.nitems, a.nsplit, a.is_split, s),
          "planning items");
@@ -1075,7 +1075,7 @@ static asohf_status enqueue_hier(asohf_ctx* ctx, const PipeArgs& a, hipStream_t 
   HIPCHK(launch_cursor_reset(a.start, a.cursor, a.nkeys, a.cus, s), "cursors");
   HIPCHK(launch_bin_base_tsc(a.P, a.A, a.cursor, a.start, const_cast<int4*>(a.D.entries), a.C, a.BT, a.cus, s),
          "base-grid binning");
-  if (a.A.n > 0) HIPCHK(launch_reduce_blocks(a.BT, hier_bin_blocks(a.A.n), a.C, s), "block sums");
+  if (a.A.n > 0) HIPCHK(launch_reduce_blocks(a.BT, hier_bin_blocks(a.A.n), a.C, a.cus, s), "block sums");
   if (events) {
     HIPCHK(hipEventRecord(ctx->ev[1], s), "event");
     HIPCHK(hipEventRecord(ctx->ev[2], s), "event");
@@ -1094,7 +1094,7 @@ static asohf_status enqueue_hier(asohf_ctx* ctx, const PipeArgs& a, hipStream_t 
   HIPCHK(launch_split_reduce(a.P, a.D, a.red, a.cus, s), "split tiles");
   if (events) HIPCHK(hipEventRecord(ctx->ev[5], s), "event");
   if (a.grid_mass) HIPCHK(launch_grid_sums(a.P, a.D, a.red, a.grid_mass, s), "grid sums");
-  HIPCHK(launch_hier_stats(a.item_pairs, a.n_items, a.start, a.cursor, a.nkeys, a.C, s), "statistics");
+  HIPCHK(launch_hier_stats(a.item_pairs, a.n_items, a.start, a.cursor, a.nkeys, a.C, a.cus, s), "statistics");
   return ASOHF_OK;
 }
 

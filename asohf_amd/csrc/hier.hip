@@ -541,14 +541,17 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
 }
 
 // statistics of the call: members = base members + the members every emission
-// item found; entries = the filled part of every tile's range
-__global__ void __launch_bounds__(1024) hier_stats(const unsigned* item_pairs, long long n_items,
-                                                  const unsigned long long* start,
-                                                  const unsigned long long* cursor, long long nkeys, Counters* C) {
-  __shared__ unsigned long long sp[16], se[16];
+// item found; entries = the filled part of every tile's range.  Every
+// workgroup adds its grid-stride slice to the counters (cleared at the start
+// of the call; integer sums, so the order does not matter).
+__global__ void __launch_bounds__(256) hier_stats(const unsigned* item_pairs, long long n_items,
+                                                 const unsigned long long* start,
+                                                 const unsigned long long* cursor, long long nkeys, Counters* C) {
+  __shared__ unsigned long long sp[4], se[4];
   unsigned long long p = 0, e = 0;
-  for (long long i = threadIdx.x; i < n_items; i += blockDim.x) p += item_pairs[i];
-  for (long long k = threadIdx.x; k < nkeys; k += blockDim.x) e += cursor[k] - start[k];
+  const long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x, di = (long long)gridDim.x * blockDim.x;
+  for (long long i = i0; i < n_items; i += di) p += item_pairs[i];
+  for (long long k = i0; k < nkeys; k += di) e += cursor[k] - start[k];
   p = wave_sum(p);
   e = wave_sum(e);
   if ((threadIdx.x & 63) == 0) { sp[threadIdx.x >> 6] = p; se[threadIdx.x >> 6] = e; }
@@ -556,8 +559,8 @@ __global__ void __launch_bounds__(1024) hier_stats(const unsigned* item_pairs, l
   if (threadIdx.x == 0) {
     unsigned long long tp = 0, te = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { tp += sp[w]; te += se[w]; }
-    C->n_pairs += tp;
-    C->n_entries = te;
+    if (tp) atomicAdd(&C->n_pairs, tp);
+    if (te) atomicAdd(&C->n_entries, te);
   }
 }
 
@@ -617,8 +620,12 @@ hipError_t launch_emit_tsc(const Plan& P, const Item* items, long long item_lo, 
 }
 
 hipError_t launch_hier_stats(const unsigned* item_pairs, long long n_items, const unsigned long long* start,
-                             const unsigned long long* cursor, long long nkeys, Counters* C, hipStream_t s) {
-  hier_stats<<<1, 1024, 0, s>>>(item_pairs, n_items, start, cursor, nkeys, C);
+                             const unsigned long long* cursor, long long nkeys, Counters* C, int cus,
+                             hipStream_t s) {
+  const long long want = (std::max(n_items, nkeys) + 255) / 256;
+  if (want <= 0) return hipSuccess;
+  hier_stats<<<(int)std::min<long long>(want, (long long)std::max(cus, 1) * 4), 256, 0, s>>>(item_pairs, n_items, start,
+                                                                                          cursor, nkeys, C);
   return hipGetLastError();
 }
 

@@ -208,12 +208,19 @@ __global__ void __launch_bounds__(kBinThreads) count_kernel(Plan P, Parts A, uns
   }
 }
 
-__global__ void __launch_bounds__(1024) reduce_blocks(BlockTables BT, long long nblk, Counters* C) {
-  __shared__ unsigned long long sp[16];
-  __shared__ float sm[16];
+// The call's base members and largest mass from the per-block partials:
+// every workgroup folds a grid-stride slice and adds its part to the
+// counters (cleared at the start of the call); integer sums and the maximum
+// do not depend on the order.  The mass maximum compares FP32 bit patterns,
+// which order like the values for masses >= 0; the sign bit is cleared so a
+// -0 mass counts as 0.
+__global__ void __launch_bounds__(256) reduce_blocks(BlockTables BT, long long nblk, Counters* C) {
+  __shared__ unsigned long long sp[4];
+  __shared__ float sm[4];
   unsigned long long p = 0;
   float mm = 0.f;
-  for (long long b = threadIdx.x; b < nblk; b += blockDim.x) {
+  for (long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x; b < nblk;
+       b += (long long)gridDim.x * blockDim.x) {
     p += BT.pairs[b];
     mm = fmaxf(mm, __uint_as_float(BT.mmax[b]));
   }
@@ -223,8 +230,8 @@ __global__ void __launch_bounds__(1024) reduce_blocks(BlockTables BT, long long 
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { p = sp[0] + sp[w]; sp[0] = p; sm[0] = fmaxf(sm[0], sm[w]); }
-    C->n_pairs = sp[0];
-    C->mmax_bits = __float_as_uint(sm[0]);
+    if (sp[0]) atomicAdd(&C->n_pairs, sp[0]);
+    atomicMax(&C->mmax_bits, __float_as_uint(sm[0]) & 0x7fffffffu);
   }
 }
 
@@ -1621,8 +1628,10 @@ hipError_t launch_scan(const unsigned* in, long long n, unsigned long long* bsum
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_blocks(const BlockTables& BT, long long nblk, Counters* C, hipStream_t s) {
-  reduce_blocks<<<1, 1024, 0, s>>>(BT, nblk, C);
+hipError_t launch_reduce_blocks(const BlockTables& BT, long long nblk, Counters* C, int cus, hipStream_t s) {
+  if (nblk <= 0) return hipSuccess;
+  const long long want = (nblk + 255) / 256;
+  reduce_blocks<<<(int)std::min<long long>(want, (long long)std::max(cus, 1) * 4), 256, 0, s>>>(BT, nblk, C);
   return hipGetLastError();
 }
 

@@ -23,7 +23,7 @@ hipError_t launch_membership(const Plan& P, const Parts& A, int* out, long long 
 long long scan_scratch_elems(long long n);
 hipError_t launch_scan(const unsigned* in, long long n, unsigned long long* bsum, unsigned long long* start,
                        unsigned long long* cursor, hipStream_t s);
-hipError_t launch_reduce_blocks(const BlockTables& BT, long long nblk, Counters* C, hipStream_t s);
+hipError_t launch_reduce_blocks(const BlockTables& BT, long long nblk, Counters* C, int cus, hipStream_t s);
 hipError_t launch_plan_count(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
                              unsigned* nitems_tile, unsigned* nsplit_tile, unsigned* is_split, hipStream_t s);
 hipError_t launch_plan_write(const unsigned long long* start, int ntiles, int chunk, int chunk_wide,
@@ -49,7 +49,8 @@ hipError_t launch_emit_tsc(const Plan& P, const Item* items, long long item_lo, 
                            unsigned long long* cursor, const unsigned long long* start, int4* entries,
                            unsigned* item_pairs, Counters* C, int cus, hipStream_t s);
 hipError_t launch_hier_stats(const unsigned* item_pairs, long long n_items, const unsigned long long* start,
-                             const unsigned long long* cursor, long long nkeys, Counters* C, hipStream_t s);
+                             const unsigned long long* cursor, long long nkeys, Counters* C, int cus,
+                             hipStream_t s);
 hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
                           float* dx, float* dy, float* dz, const unsigned* order, int cus, hipStream_t s);
 hipError_t launch_cell_keys(const Plan& P, const Parts& A, unsigned* key, unsigned* idx, int cus, hipStream_t s);
