@@ -1143,14 +1143,16 @@ __global__ void __launch_bounds__(1024) deposit_kernel(Plan P, DepArgs D) {
 }
 
 // Split tiles: the accumulated interiors normalised and written as the
-// deposit's writer does.  Work unit = (split tile, 256 consecutive cells), so
-// a heavy tile is spread over many workgroups; the exact int64 (FP64 in
-// atomic mode) tile mass is accumulated per unit.
+// deposit's writer does.  Work unit = (split tile, kPer * 256 consecutive
+// cells, kPer per thread with every load issued before the first store), so a
+// heavy tile is spread over many workgroups; the exact int64 (FP64 in atomic
+// mode) tile mass is accumulated per unit.
 template <int RED, int TX, int TY, int TZ>
 __global__ void __launch_bounds__(256) split_reduce(Plan P, DepArgs D) {
   using Acc = typename AccT<RED>::T;
   constexpr int T = TX * TY * TZ;
-  constexpr int kCells = 256;
+  constexpr int kPer = 4;
+  constexpr int kCells = 256 * kPer;
   constexpr int cpt = (T + kCells - 1) / kCells;
   __shared__ long long s_redi[4];
   __shared__ double s_redd[4];
@@ -1159,7 +1161,7 @@ __global__ void __launch_bounds__(256) split_reduce(Plan P, DepArgs D) {
   const unsigned long long units = *D.n_split_tiles * cpt;
   for (unsigned long long u = blockIdx.x; u < units; u += gridDim.x) {
     const int t = D.split_tiles[u / cpt];
-    const int c = (int)(u % cpt) * kCells + threadIdx.x;
+    const int c0 = (int)(u % cpt) * kCells + threadIdx.x;
     const int g = P.tile_grid[t];
     const GridDesc& G = P.grids[g];
     const int local = t - G.tile0;
@@ -1167,17 +1169,28 @@ __global__ void __launch_bounds__(256) split_reduce(Plan P, DepArgs D) {
     const int i0 = tx * TX + 1, j0 = ty * TY + 1, k0c = tz * TZ + 1;
     const int ex = min(TX, G.n[0] - i0 + 1), ey = min(TY, G.n[1] - j0 + 1), ez = min(TZ, G.n[2] - k0c + 1);
     const Acc* part = reinterpret_cast<const Acc*>(D.scratch) + (long long)D.split_slot[t] * T;
+    float* out = (g == 0 ? D.u1 : D.u11) + G.off;
+    Acc v[kPer];
+    bool in[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int c = c0 + r * 256;
+      const int a = c % TX, b = (c / TX) % TY, cz = c / (TX * TY);
+      in[r] = c < T && a < ex && b < ey && cz < ez;
+      v[r] = in[r] ? part[c] : (Acc)0;
+    }
     long long msi = 0;
     double msd = 0.0;
-    const int a = c % TX, b = (c / TX) % TY, cz = c / (TX * TY);
-    if (c < T && a < ex && b < ey && cz < ez) {
-      const Acc v = part[c];
-      if (RED == 0) msi = (long long)v;
-      else msd = (double)v;
-      float f = RED == 0 ? (float)((double)v * inv_scale) : (float)v;
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      if (!in[r]) continue;
+      const int c = c0 + r * 256;
+      const int a = c % TX, b = (c / TX) % TY, cz = c / (TX * TY);
+      if (RED == 0) msi += (long long)v[r];
+      else msd += (double)v[r];
+      float f = RED == 0 ? (float)((double)v[r] * inv_scale) : (float)v[r];
       f = f / G.den;
       f = f / P.fdm;
-      float* out = (g == 0 ? D.u1 : D.u11) + G.off;
       out[(long long)(i0 - 1 + a) + (long long)G.ld1 * (j0 - 1 + b) + (long long)G.ld2 * (k0c - 1 + cz)] = f;
     }
     if (RED == 0) {
