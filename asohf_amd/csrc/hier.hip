@@ -194,7 +194,7 @@ __device__ __forceinline__ void reserve_round(const int* keys, unsigned* vals, u
       const unsigned long long b0 = atomicAdd(&cursor[key], (unsigned long long)v);
       const unsigned long long lim = start[key + 1];
       const unsigned long long room = b0 >= lim ? 0ull : (lim - b0 < v ? lim - b0 : (unsigned long long)v);
-      base[k] = b0 | (room << kSlotBits);
+      base[k] = (b0 & kSlotMask) | (room << kSlotBits);   // room 0 when b0 is past the capacity
       if (b0 + v > lim) set_overflow(C);
     }
     vals[k] = 0u;
