@@ -116,30 +116,30 @@ __global__ void __launch_bounds__(256) count_to_capacity(unsigned* cnt, long lon
   }
 }
 
-constexpr int kGroups = 4;   // keys of a wave grouped with ballots before lanes add one by one
-constexpr int kEmGroups = 4; // the same in the emission pass
+constexpr int kGroups = 4;   // keys of a wave grouped with ballots; the other lanes add one by one
 
 // The position of one emission per lane that wants one (key = destination
 // tile key) in its key's range of the workgroup's round: the first kGroups
 // keys of the wave are grouped with ballots (no memory operation); each
 // group's first lane finds its key's table slot and adds the group's size to
-// the key's count, all groups' first lanes at once, and the old count plus
-// a lane's rank in its group is the lane's position; lanes left over look up
-// their key and add 1 each.  h = -1 when the table is full (the emission is
-// then not counted).  Every lane of the wave must be active.
-template <int CAP, int NG = kGroups>
+// the key's count, and the old count plus a lane's rank in its group is the
+// lane's position; lanes left over look up their key and add 1 each, in the
+// same table operations as the groups' first lanes.  h = -1 when the table
+// is full (the emission is then not counted).  Every lane of the wave must
+// be active.
+template <int CAP>
 __device__ __forceinline__ void count_round(int* keys, unsigned* vals, bool want, int key, int& h, unsigned& at) {
   const int lane = (int)__lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   h = -1;
   at = 0;
   unsigned long long pend = __ballot(want);
-  int lead[NG];
-  unsigned long long gm[NG];
+  int lead[kGroups];
+  unsigned long long gm[kGroups];
   int ng = 0;
   unsigned mine = 0;   // the group size when this lane leads a group
 #pragma unroll
-  for (int gi = 0; gi < NG; ++gi) {
+  for (int gi = 0; gi < kGroups; ++gi) {
     lead[gi] = 0;
     gm[gi] = 0ull;
     if (pend) {
@@ -153,6 +153,9 @@ __device__ __forceinline__ void count_round(int* keys, unsigned* vals, bool want
       ng = gi + 1;
     }
   }
+  // the lanes left over add 1 each, in the same table operations
+  const bool left = (pend >> lane) & 1ull;
+  if (left) mine = 1u;
   int hh = -1;
   unsigned old = 0;
   if (mine) {
@@ -160,7 +163,7 @@ __device__ __forceinline__ void count_round(int* keys, unsigned* vals, bool want
     if (hh >= 0) old = atomicAdd(&vals[hh], mine);
   }
 #pragma unroll
-  for (int gi = 0; gi < NG; ++gi) {
+  for (int gi = 0; gi < kGroups; ++gi) {
     if (gi < ng) {
       const int hg = __builtin_amdgcn_readlane(hh, lead[gi]);
       const unsigned og = (unsigned)__builtin_amdgcn_readlane((int)old, lead[gi]);
@@ -170,45 +173,9 @@ __device__ __forceinline__ void count_round(int* keys, unsigned* vals, bool want
       }
     }
   }
-  if ((pend >> lane) & 1ull) {
-    h = slot_insert<CAP>(keys, key);
-    if (h >= 0) at = atomicAdd(&vals[h], 1u);
-  }
-}
-
-// count_round with the groups' table operations one group after another,
-// each group's first lane finding the key's slot and adding the group's size:
-// cheaper where most groups are single lanes (the base binning's scattered
-// keys).  Same results and requirements as count_round.
-template <int CAP>
-__device__ __forceinline__ void count_round_seq(int* keys, unsigned* vals, bool want, int key, int& h, unsigned& at) {
-  const int lane = (int)__lane_id();
-  const unsigned long long below = (1ull << lane) - 1ull;
-  h = -1;
-  at = 0;
-  unsigned long long pend = __ballot(want);
-  for (int gi = 0; gi < kGroups && pend; ++gi) {
-    const int l0 = __ffsll((long long)pend) - 1;
-    const int k0 = __builtin_amdgcn_readlane(key, l0);
-    const bool mine = want && key == k0;
-    const unsigned long long mm = __ballot(mine);
-    int hh = 0;
-    unsigned old = 0;
-    if (lane == l0) {
-      hh = slot_insert<CAP>(keys, k0);
-      if (hh >= 0) old = atomicAdd(&vals[hh], (unsigned)__popcll(mm));
-    }
-    hh = __builtin_amdgcn_readlane(hh, l0);
-    old = (unsigned)__builtin_amdgcn_readlane((int)old, l0);
-    if (mine) {
-      h = hh;
-      at = old + (unsigned)__popcll(mm & below);
-    }
-    pend &= ~mm;
-  }
-  if ((pend >> lane) & 1ull) {
-    h = slot_insert<CAP>(keys, key);
-    if (h >= 0) at = atomicAdd(&vals[h], 1u);
+  if (left) {
+    h = hh;
+    at = old;
   }
 }
 
@@ -361,7 +328,7 @@ __global__ void __launch_bounds__(kHbThreads) bin_base_tsc(Plan P, Parts A, unsi
       const unsigned long long w = want ? wl[idx] : 0ull;
       int h;
       unsigned at;
-      count_round_seq<kHbCap>(keys, vals, want, (int)(unsigned)w, h, at);
+      count_round<kHbCap>(keys, vals, want, (int)(unsigned)w, h, at);
       if (want) {
         if (h >= 0) {
           wl[idx] = (w & 0xffffffff00000000ull) | ((unsigned long long)h << 16) | at;
@@ -551,7 +518,7 @@ __global__ void __launch_bounds__(kEmThreads, 5) emit_tsc(Plan P, const Item* it
             const unsigned long long w = want ? wl[idx] : 0ull;
             int h;
             unsigned at;
-            count_round<kEmCap, kEmGroups>(keys, vals, want, (int)(unsigned)w, h, at);
+            count_round<kEmCap>(keys, vals, want, (int)(unsigned)w, h, at);
             if (want) {
               if (h >= 0) {
                 wl[idx] = (w & 0xffffffff00000000ull) | ((unsigned long long)h << 16) | at;
