@@ -116,66 +116,20 @@ __global__ void __launch_bounds__(256) count_to_capacity(unsigned* cnt, long lon
   }
 }
 
-constexpr int kGroups = 4;   // keys of a wave grouped with ballots; the other lanes add one by one
-
 // The position of one emission per lane that wants one (key = destination
-// tile key) in its key's range of the workgroup's round: the first kGroups
-// keys of the wave are grouped with ballots (no memory operation); each
-// group's first lane finds its key's table slot and adds the group's size to
-// the key's count, and the old count plus a lane's rank in its group is the
-// lane's position; lanes left over look up their key and add 1 each, in the
-// same table operations as the groups' first lanes.  h = -1 when the table
-// is full (the emission is then not counted).  Every lane of the wave must
-// be active.
+// tile key) in its key's range of the workgroup's round: the lane finds the
+// key's table slot and adds 1 to its count, whose old value is the position.
+// Lanes of a wave with the same key meet at one LDS address, which the LDS
+// serialises at less cost than grouping the lanes first (ballots and lane
+// reads per group, measured slower with 1 to 8 groups).  h = -1 when the
+// table is full (the emission is then not counted).
 template <int CAP>
 __device__ __forceinline__ void count_round(int* keys, unsigned* vals, bool want, int key, int& h, unsigned& at) {
-  const int lane = (int)__lane_id();
-  const unsigned long long below = (1ull << lane) - 1ull;
   h = -1;
   at = 0;
-  unsigned long long pend = __ballot(want);
-  int lead[kGroups];
-  unsigned long long gm[kGroups];
-  int ng = 0;
-  unsigned mine = 0;   // the group size when this lane leads a group
-#pragma unroll
-  for (int gi = 0; gi < kGroups; ++gi) {
-    lead[gi] = 0;
-    gm[gi] = 0ull;
-    if (pend) {
-      const int l0 = __ffsll((long long)pend) - 1;
-      const int k0 = __builtin_amdgcn_readlane(key, l0);
-      const unsigned long long mm = __ballot(want && key == k0);
-      lead[gi] = l0;
-      gm[gi] = mm;
-      if (lane == l0) mine = (unsigned)__popcll(mm);
-      pend &= ~mm;
-      ng = gi + 1;
-    }
-  }
-  // the lanes left over add 1 each, in the same table operations
-  const bool left = (pend >> lane) & 1ull;
-  if (left) mine = 1u;
-  int hh = -1;
-  unsigned old = 0;
-  if (mine) {
-    hh = slot_insert<CAP>(keys, key);
-    if (hh >= 0) old = atomicAdd(&vals[hh], mine);
-  }
-#pragma unroll
-  for (int gi = 0; gi < kGroups; ++gi) {
-    if (gi < ng) {
-      const int hg = __builtin_amdgcn_readlane(hh, lead[gi]);
-      const unsigned og = (unsigned)__builtin_amdgcn_readlane((int)old, lead[gi]);
-      if ((gm[gi] >> lane) & 1ull) {
-        h = hg;
-        at = og + (unsigned)__popcll(gm[gi] & below);
-      }
-    }
-  }
-  if (left) {
-    h = hh;
-    at = old;
+  if (want) {
+    h = slot_insert<CAP>(keys, key);
+    if (h >= 0) at = atomicAdd(&vals[h], 1u);
   }
 }
 
