@@ -1467,11 +1467,13 @@ asohf_status asohf_hip_density_adjoint(asohf_ctx* ctx, const asohf_geom* geom, i
   // the forward's input checks (non-finite values, negative masses,
   // positions outside the TSC base range), read back after the gather
   HIPCHK(launch_fill_u32((unsigned*)C, 0u, sizeof(Counters) / 4, s), "clearing counters");
-  if (n > 0) HIPCHK(launch_plain_validate(ctx->plan.dev, A, C, ctx->cus, s), "validation");
   // Large inputs are gathered in base-cell order (a radix sort of (cell,
-  // index) pairs): neighbouring lanes then read the same gradient rows.
+  // index) pairs): neighbouring lanes then read the same gradient rows.  The
+  // key pass makes the input checks; smaller inputs get the validation pass.
+  const bool sorted = n >= kAdjointSortMin && n < (1ll << 32);
+  if (n > 0 && !sorted) HIPCHK(launch_plain_validate(ctx->plan.dev, A, C, ctx->cus, s), "validation");
   const unsigned* order = nullptr;
-  if (n >= kAdjointSortMin && n < (1ll << 32)) {
+  if (sorted) {
     const long long nc = (long long)geom->nx * geom->ny * geom->nz;
     int bits = 1;
     while (bits < 32 && (1ll << bits) < nc) ++bits;
@@ -1484,7 +1486,7 @@ asohf_status asohf_hip_density_adjoint(asohf_ctx* ctx, const asohf_geom* geom, i
     HIPCHK(sort_cell_pairs(nullptr, tb, nullptr, nullptr, nullptr, nullptr, (unsigned long long)n, bits, s),
            "sizing the sort");
     HIPCHK(ctx->ord_tmp.ensure(std::max<size_t>(tb, 1)), "allocating the sort");
-    HIPCHK(launch_cell_keys(ctx->plan.dev, A, ctx->ord[0].as<unsigned>(), ctx->ord[1].as<unsigned>(), ctx->cus, s),
+    HIPCHK(launch_cell_keys(ctx->plan.dev, A, ctx->ord[0].as<unsigned>(), ctx->ord[1].as<unsigned>(), C, ctx->cus, s),
            "cell keys");
     HIPCHK(sort_cell_pairs(ctx->ord_tmp.p, tb, ctx->ord[0].as<unsigned>(), ctx->ord[2].as<unsigned>(),
                            ctx->ord[1].as<unsigned>(), ctx->ord[3].as<unsigned>(), (unsigned long long)n, bits, s),

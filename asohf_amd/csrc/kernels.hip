@@ -1259,18 +1259,32 @@ __device__ __forceinline__ void load_row3(const float* p, float out[3]) {
 
 // Base-cell key (x fastest) of every particle, for the gather order of the
 // adjoint: lanes of a wave that take particles of one cell read the same
-// rows of the upstream gradient.
-__global__ void __launch_bounds__(256) cell_keys(Plan P, Parts A, unsigned* key, unsigned* idx) {
+// rows of the upstream gradient.  The pass also makes the TSC input checks
+// (non-finite values, negative masses, positions outside the base range:
+// the flags of the forward's validation), as it reads the positions anyway.
+__global__ void __launch_bounds__(256) cell_keys(Plan P, Parts A, unsigned* key, unsigned* idx, Counters* C) {
   const GridDesc& G = P.grids[0];
+  unsigned err = 0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < A.n;
        i += (long long)gridDim.x * blockDim.x) {
     const float p[3] = {A.x[i], A.y[i], A.z[i]};
+    const float m = A.m[i];
+    if (!(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]) && isfinite(m))) {
+      err |= kErrNonFinite;
+    } else {
+      if (m < 0.f) err |= kErrNegMass;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        if (!tsc_base_in_range(p[a], P.xl, G.inv_d[a], G.n[a])) err |= kErrWrap;
+    }
     int c[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) c[a] = clampi(tsc_base_index(p[a], P.xl, G.d[a], G.inv_d[a], G.n[a]), 1, G.n[a]) - 1;
     key[i] = (unsigned)c[0] + (unsigned)G.n[0] * ((unsigned)c[1] + (unsigned)G.n[1] * (unsigned)c[2]);
     idx[i] = (unsigned)i;
   }
+  err = wave_or(err);
+  if ((threadIdx.x & 63) == 0 && err) atomicOr(&C->err, err);
 }
 
 // One (particle, grid) term of the adjoint: the nine rows of three gradient
@@ -1293,12 +1307,13 @@ struct AdjAcc {
 
 // One thread per particle, in the given order (NULL: index order).  The
 // result of a particle does not depend on the order.
-__global__ void __launch_bounds__(256) adjoint_gather(Plan P, Parts A, const float* g1, const float* g11,
+__global__ void __launch_bounds__(256, 8) adjoint_gather(Plan P, Parts A, const float* g1, const float* g11,
                                                       float* dm, float* dxo, float* dyo, float* dzo,
                                                       const unsigned* order) {
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < A.n;
        t += (long long)gridDim.x * blockDim.x) {
     const long long i = order ? (long long)order[t] : t;
+    if (i >= A.n) continue;   // the order is a permutation of [0, n); checked so a bad one cannot read out of range
     const float p[3] = {A.x[i], A.y[i], A.z[i]};
     const float m = A.m[i];
     AdjAcc acc;
@@ -1757,8 +1772,9 @@ hipError_t launch_grid_sums(const Plan& P, const DepArgs& D, int reduction, doub
   return hipGetLastError();
 }
 
-hipError_t launch_cell_keys(const Plan& P, const Parts& A, unsigned* key, unsigned* idx, int cus, hipStream_t s) {
-  cell_keys<<<grid_for(A.n, 256, cus * 16), 256, 0, s>>>(P, A, key, idx);
+hipError_t launch_cell_keys(const Plan& P, const Parts& A, unsigned* key, unsigned* idx, Counters* C, int cus,
+                            hipStream_t s) {
+  cell_keys<<<grid_for(A.n, 256, cus * 16), 256, 0, s>>>(P, A, key, idx, C);
   return hipGetLastError();
 }
 

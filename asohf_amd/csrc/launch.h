@@ -53,7 +53,8 @@ hipError_t launch_hier_stats(const unsigned* item_pairs, long long n_items, cons
                              hipStream_t s);
 hipError_t launch_adjoint(const Plan& P, const Parts& A, const float* g1, const float* g11, float* dm,
                           float* dx, float* dy, float* dz, const unsigned* order, int cus, hipStream_t s);
-hipError_t launch_cell_keys(const Plan& P, const Parts& A, unsigned* key, unsigned* idx, int cus, hipStream_t s);
+hipError_t launch_cell_keys(const Plan& P, const Parts& A, unsigned* key, unsigned* idx, Counters* C, int cus,
+                            hipStream_t s);
 hipError_t species_sort_bytes(unsigned long long n, size_t& bytes);
 hipError_t launch_species(const int nmax[3], const float d[3], float bas, long long n, const float* x,
                           const float* y, const float* z, const float* m, int* species, long long* order,

@@ -13,8 +13,14 @@ namespace asohf {
 hipError_t sort_cell_pairs(void* temp, size_t& temp_bytes, const unsigned* key_in, unsigned* key_out,
                            const unsigned* idx_in, unsigned* idx_out, unsigned long long n, int end_bit,
                            hipStream_t s) {
-  return rocprim::radix_sort_pairs(temp, temp_bytes, key_in, key_out, idx_in, idx_out, n, 0u,
-                                   (unsigned)end_bit, s);
+  // 10 bits per onesweep pass: a 30-bit cell key of a 1024^3 grid in three
+  // passes instead of four
+  using Cfg = rocprim::radix_sort_config<
+      rocprim::default_config, rocprim::default_config,
+      rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 10,
+                                          rocprim::block_radix_rank_algorithm::match>>;
+  return rocprim::radix_sort_pairs<Cfg>(temp, temp_bytes, key_in, key_out, idx_in, idx_out, n, 0u,
+                                        (unsigned)end_bit, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -156,3 +156,33 @@ def test_autograd_vs_dense_torch(bld):
     for got, want in ((x.grad, xc.grad), (y.grad, yc.grad), (z.grad, zc.grad), (m.grad, mc.grad)):
         got, want = got.cpu().numpy(), want.numpy()
         assert np.max(np.abs(got - want)) <= 1e-4 * np.abs(want).max()
+
+
+def test_adjoint_rejects_invalid_input_in_gather_order(bld):
+    """From 2^20 particles the adjoint gathers in base-cell order and its key
+    pass makes the input checks: a NaN position, a negative mass and a
+    position far outside the base grid are each rejected; the same particles
+    without them pass with finite gradients."""
+    from asohf_amd._lib import AsohfError
+    snap, _ = load_fixture("seeded_12")
+    g = snap.geom
+    n = (1 << 20) + 7
+    rng = np.random.default_rng(5)
+    L = float(g.lado0)
+    x0, y0, z0 = (((rng.random(n) * 0.98 - 0.49) * L).astype(np.float32) for _ in range(3))
+    m0 = np.full(n, snap.m[0], np.float32)
+    w = torch.zeros((g.nz, g.ny, g.nx), device="cuda:0")
+    for case in ("nan", "negative mass", "far"):
+        print(f"case adjoint invalid input {case} n={n}", flush=True)
+        x, y, z, m = x0.copy(), y0.copy(), z0.copy(), m0.copy()
+        if case == "nan":
+            x[n // 3] = np.nan
+        elif case == "negative mass":
+            m[n // 2] = -1.0
+        else:
+            y[n - 1] = np.float32(10 * L)
+        with pytest.raises(AsohfError):
+            bld.adjoint(g, *dev(x, y, z, m), w, None)
+    outs = bld.adjoint(g, *dev(x0, y0, z0, m0), w + 1.0, None)
+    for t in outs:
+        assert torch.isfinite(t).all()
