@@ -1306,7 +1306,9 @@ struct AdjAcc {
 };
 
 // One thread per particle, in the given order (NULL: index order).  The
-// result of a particle does not depend on the order.
+// result of a particle does not depend on the order.  Eight workgroups per
+// CU (at most 64 VGPRs, a few of them spilled): the gather waits on memory,
+// and the extra waves in flight gain more than the spills cost.
 __global__ void __launch_bounds__(256, 8) adjoint_gather(Plan P, Parts A, const float* g1, const float* g11,
                                                       float* dm, float* dxo, float* dyo, float* dzo,
                                                       const unsigned* order) {
